@@ -1,0 +1,210 @@
+/*
+ * phs.h -- C ABI of libphs.so, the MI355X (gfx950) simulation hot path of
+ * PyHeartSim: RK4 Mitchell-Schaeffer / Aliev-Panfilov reaction,
+ * Crank-Nicolson diffusion on the voxel lattice, per-voxel activation times,
+ * 1 kHz sampling, and the unipolar electrogram.
+ *
+ * The reference has no FFI: its boundary is two Python functions,
+ *   simulation/simulation_gpu.py:261          compute(n_voxel, P_2d, geometry_data,
+ *                                              simulation_parameters, arrhythmia_parameters)
+ *   simulation/unipolar_electrogram_gpu.py:71 compute(electrode_xyz, voxel, D0, c_voxel,
+ *                                              action_potential, Delta, neighbor_id_2d, batch_size)
+ * The Python package pyheartsim_amd.simulation keeps those two signatures and binds this
+ * header through ctypes.  Each entry point below names the reference code it replaces.
+ *
+ * Conventions
+ *   - Return 0 on success, a negative PHS_ERR_* code otherwise; phs_last_error() gives the
+ *     message (thread-local).  No C++ exception crosses this boundary.
+ *   - Pointers documented "host" are read during the call only.  Pointers documented
+ *     "device" are caller-owned HBM buffers that must stay valid while the library uses them.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).  Every call that
+ *     takes a stream is ordered on it.
+ *   - Handles are not thread-safe: use one handle per host thread.
+ */
+#ifndef PHS_H
+#define PHS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHS_OK 0
+#define PHS_ERR_ARG (-1)         /* invalid argument */
+#define PHS_ERR_HIP (-2)         /* HIP runtime error */
+#define PHS_ERR_OOM (-3)         /* device allocation failed */
+#define PHS_ERR_UNSUPPORTED (-4) /* input outside the supported physics (e.g. anisotropic D) */
+#define PHS_ERR_CAPACITY (-5)    /* arena has no room for the simulation */
+#define PHS_ERR_STATE (-6)       /* call not valid in the handle's current state */
+
+/* Voxel flag bits (vflags, one byte per voxel). */
+#define PHS_VF_SCAR 0x01       /* node_flag 4: non-conductive for the whole run */
+#define PHS_VF_TEMP_BLOCK 0x02 /* node_flag 3: non-conductive while schedule bit 2 is set */
+#define PHS_VF_S1 0x04         /* S1 pacing site */
+#define PHS_VF_S2 0x08         /* S2 pacing site */
+
+/* Schedule bits (one byte per step). */
+#define PHS_SCHED_S1 0x01
+#define PHS_SCHED_S2 0x02
+#define PHS_SCHED_TEMP_BLOCK 0x04
+
+/* One simulation, all host arrays in the caller's (original) voxel order.
+ * Mirrors what simulation/simulation_gpu.py:261-357 reads from its five arguments:
+ *   model, params[5]  <- heart_model_flag, P_2d[:, 15:20] (uniform over voxels)
+ *   c, c_voxel        <- P_2d[:, 20] (c_voxel NULL when uniform; scar handled by vflags)
+ *   dt, delta         <- simulation_parameters['dt'], geometry_data['Delta']
+ *   nb                <- geometry_data['neighbor_id_2d'] columns 0-5 (row stride nb_stride)
+ *   vflags            <- arrhythmia_parameters node_flag (3, 4) and the S1/S2 site lists
+ *   schedule          <- simulation/pacing.py:136-170 and the 350-step temporary block,
+ *                        evaluated in FP64 on the host, one byte per step
+ *   steps_per_sample, n_samples  <- :286-290, :356, :395-407 (1 kHz sampling) */
+typedef struct phs_sim_desc {
+  int32_t n_voxel;
+  int32_t model;           /* 0 = Mitchell-Schaeffer, 1 = Aliev-Panfilov */
+  double params[5];        /* MS: tau_open, tau_close, tau_in, tau_out, v_gate;
+                              AP: k, a, epsilon_0, mu1, mu2 */
+  double c;                /* conductivity of conducting voxels */
+  const double* c_voxel;   /* host [n_voxel] or NULL (uniform c) */
+  double dt;
+  double delta;
+  int64_t n_steps;         /* length of `schedule` */
+  int32_t steps_per_sample;/* save after step s when s % steps_per_sample == 0 (0 = every step) */
+  int32_t n_samples;       /* capacity of the sample outputs */
+  const int32_t* ijk;      /* host [n_voxel][3] integer lattice coordinates (bricked form) */
+  const int32_t* nb;       /* host [n_voxel][nb_stride] neighbour ids, columns 0-5 used, -1 = none */
+  int32_t nb_stride;
+  const uint8_t* vflags;   /* host [n_voxel] PHS_VF_* */
+  const uint8_t* schedule; /* host [n_steps] PHS_SCHED_* */
+  const float* u0;         /* host [n_voxel] initial u or NULL (rest: 0) */
+  const float* h0;         /* host [n_voxel] initial h or NULL (rest: MS 1, AP 0) */
+  double lat_threshold;    /* activation threshold for the per-voxel LAT (0.13) */
+} phs_sim_desc;
+
+/* Caller-owned device outputs of one simulation. */
+typedef struct phs_outputs {
+  void* samples_u;         /* device [n_samples][n_voxel] or NULL; float, or double for the
+                              plain FP64 form when sample_fp64 = 1 */
+  void* samples_h;         /* device, same layout, or NULL */
+  int32_t sample_fp64;     /* plain FP64 form only: samples are double */
+  int32_t* lat;            /* device [max_beats][n_voxel] step index of each upward crossing,
+                              -1 = none (the library initialises it) */
+  int32_t max_beats;       /* <= 7 */
+  void* probe_u;           /* device [n_steps][n_probes] u after every step, or NULL (float;
+                              double for the plain FP64 form when sample_fp64 = 1) */
+  const int32_t* probe_ids;/* host [n_probes] voxel ids */
+  int32_t n_probes;
+} phs_outputs;
+
+int phs_version(void);
+const char* phs_last_error(void);
+int phs_device_count(int32_t* count);
+
+/* ------------------------------------------------------------------------------------------
+ * Plain FP64 form: the reference algorithm (simulation/simulation_gpu.py:358-407 with the
+ * FP64 arithmetic of legacy/compute_simulation_cpu.py:144-228) on one GPU, voxels in the
+ * caller's order, neighbour-list operator, Chebyshev-accelerated CN solve iterated to the
+ * FP64 tolerance `tol`.  Outputs in the caller's voxel order.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct phs_plain phs_plain;
+
+/* Replaces simulation_gpu.py:296-353 (operator build, A = I - dt/2 L, buffers). */
+int phs_plain_create(const phs_sim_desc* desc, const phs_outputs* out, double tol,
+                     phs_plain** handle);
+/* Replaces one pass of the loop simulation_gpu.py:358-407 for n_steps steps. */
+int phs_plain_run(phs_plain* handle, int64_t n_steps, void* stream);
+/* Device-to-device copy of the current state (double[n_voxel] each; either may be NULL). */
+int phs_plain_state(phs_plain* handle, double* u, double* h, void* stream);
+/* Chebyshev sweeps per step and the spectral bounds used. */
+int phs_plain_info(phs_plain* handle, int32_t* sweeps, double* lambda_min, double* lambda_max);
+int phs_plain_destroy(phs_plain* handle);
+
+/* ------------------------------------------------------------------------------------------
+ * Bricked FP32 arena: many simulations of uneven size packed into one set of launches.
+ * Voxels of each simulation are reordered into Morton-ordered bricks of 64 lattice cells
+ * (one wavefront per brick).  Reaction and diffusion run in FP32; the CN right-hand side
+ * and the refinement residuals are formed in FP64.  Simulations are added and removed
+ * while others run (continuous batching); each advances by its own step counter.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct phs_arena phs_arena;
+
+typedef struct phs_arena_config {
+  int64_t max_voxels;      /* voxel capacity over all resident simulations */
+  int64_t max_bricks;      /* brick capacity */
+  int32_t max_sims;        /* slot capacity */
+  int32_t brick[3];        /* brick shape (bx, by, bz), bx*by*bz = 64 */
+  int32_t sweeps;          /* Chebyshev sweeps of the first CN solve (>= 2) */
+  int32_t refine;          /* FP64-residual refinement passes (0..4) */
+  int32_t refine_sweeps;   /* Chebyshev sweeps per refinement (>= 2) */
+  int32_t xcd_remap;       /* 1 = map consecutive bricks to one XCD */
+  int32_t diag;            /* 1 = keep u* and the correction of the last step so that
+                              phs_arena_last_residual can recompute the CN residual */
+} phs_arena_config;
+
+int phs_arena_create(const phs_arena_config* cfg, phs_arena** handle);
+/* Replaces simulation_gpu.py:296-353 for one simulation; the plan (Morton bricks) is
+ * built on the host.  Fails with PHS_ERR_CAPACITY when the arena is full. */
+int phs_arena_add(phs_arena* a, const phs_sim_desc* desc, const phs_outputs* out,
+                  int32_t* slot, void* stream);
+/* Stream-ordered: steps queued earlier on `stream` still advance the simulation. */
+int phs_arena_remove(phs_arena* a, int32_t slot, void* stream);
+/* Advance every resident simulation by n_steps steps (each stops at its own n_steps).
+ * Replaces the loop simulation_gpu.py:358-407. */
+int phs_arena_step(phs_arena* a, int32_t n_steps, void* stream);
+/* Record n_steps steps into a HIP graph once and replay it on later calls with the
+ * same n_steps (the launch pattern does not depend on which slots are resident). */
+int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream);
+/* Host-side view of a slot: steps done, voxels, bricks, first compact index. */
+int phs_arena_slot_info(phs_arena* a, int32_t slot, int64_t* steps_done, int32_t* n_voxel,
+                        int32_t* n_bricks, int64_t* voxel_offset);
+/* Compact (brick) order -> caller order map of a slot: perm[i] = caller id of voxel i. */
+int phs_arena_perm(phs_arena* a, int32_t slot, int32_t* perm_host);
+/* Device-to-device copy of a slot's current state in caller order (float[n_voxel]). */
+int phs_arena_state(phs_arena* a, int32_t slot, float* u, float* h, void* stream);
+/* Chebyshev sweeps and spectral bounds used for a slot. */
+int phs_arena_slot_bounds(phs_arena* a, int32_t slot, double* lambda_min, double* lambda_max);
+/* FP64 relative residual ||b - A x|| / ||b|| of the last CN solve of a slot, recomputed on
+ * the device from the stored correction (diagnostic; refine > 0 keeps the correction). */
+int phs_arena_last_residual(phs_arena* a, int32_t slot, double* rel_residual, void* stream);
+int phs_arena_destroy(phs_arena* a);
+
+/* Row permutation: dst[r][perm[i]] = src[r][i] for r < n_rows, i < n (float). */
+int phs_scatter_rows_f32(float* dst, const float* src, const int32_t* perm_dev, int64_t n_rows,
+                         int64_t n, void* stream);
+int phs_scatter_rows_i32(int32_t* dst, const int32_t* src, const int32_t* perm_dev,
+                         int64_t n_rows, int64_t n, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Unipolar electrogram (replaces simulation/unipolar_electrogram_gpu.py:71-152 and the
+ * kernel :21-69).  The per-voxel dipole sum
+ *     egm[t, e] = sum_v c_v / l_ve^3 * ((D_v grad u(v, t)) . l_vec_ve)
+ * is linear in u, so the central-difference gradient is folded into a lead field
+ * Wt (E x N) once per geometry and the electrogram becomes the contraction
+ *     egm^T (E x T) = Wt (E x N) . U (N x T)
+ * run on MFMA as BF16 hi/lo split products with FP32 accumulation.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct phs_egm phs_egm;
+
+/* Builds the lead field on the device.  voxel [n][3], electrodes [n_electrode][3],
+ * c_voxel [n], D [9][n] (row-major D11..D33, NULL = identity), nb [n][nb_stride] (host).
+ * `perm` (host, may be NULL) maps the sample column order to caller voxel ids: column i of
+ * the samples holds voxel perm[i]. */
+int phs_egm_create(int32_t n_voxel, int32_t n_electrode, const double* voxel,
+                   const double* electrodes, const double* c_voxel, const double* D,
+                   const int32_t* nb, int32_t nb_stride, double delta, const int32_t* perm,
+                   phs_egm** handle, void* stream);
+/* samples: device float [n_time][sample_stride] (first n_voxel columns used).
+ * egm: device float [n_time][n_electrode] (reference layout), un-normalised.
+ * precision: 0 = BF16 hi/lo split MFMA, 1 = FP32 (vector ALU), 2 = single BF16 MFMA. */
+int phs_egm_run(phs_egm* handle, const float* samples, int64_t n_time, int64_t sample_stride,
+                float* egm, int32_t precision, void* stream);
+int phs_egm_destroy(phs_egm* handle);
+/* Scale egm [n_time][n_electrode] in place so the mean peak-to-peak is `typical`
+ * (simulation/unipolar_electrogram_equation_parts.py:118-124). */
+int phs_egm_normalize(float* egm, int64_t n_time, int32_t n_electrode, double typical,
+                      void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHS_H */

@@ -1,0 +1,376 @@
+// Unipolar electrogram on MFMA.
+//
+// Reference: simulation/unipolar_electrogram_gpu.py:21-69 (one thread per (voxel, electrode),
+// FP32 atomics per sample) fed by the host gradients of
+// simulation/unipolar_electrogram_equation_parts.py:60-103 and the distance arrays of :41-58.
+//
+// The per-voxel dipole sum
+//     egm[t, e] = sum_v c_v / l^3 * sum_k (D_v^T l)_k * g_k(v, t),
+//     g_k(v, t) = (u(p_k(v), t) - u(m_k(v), t)) / (2 Delta)  when both axis neighbours exist
+// is linear in u.  Moving the central-difference stencil onto the electrode side gives a lead
+// field Wt (E x N),
+//     Wt[e, w] = 1/(2 Delta) sum_k ( ok_k(m_k(w)) W_k[e, m_k(w)] - ok_k(p_k(w)) W_k[e, p_k(w)] ),
+//     W_k[e, v] = c_v / l_ve^3 * (D_v^T l_ve)_k,  l_ve = r_v - x_e,  l = max(|l_ve|, 1 mm),
+// built once per geometry in FP64, and the electrogram becomes one contraction
+//     egm (T x E) = U (T x N) . Wt^T (N x E)
+// over the 1 kHz samples U.  It runs on MFMA (v_mfma_f32_16x16x32_bf16) as BF16 hi/lo split
+// products (hi*hi + hi*lo + lo*hi) with FP32 accumulation, split over K = N voxels with a
+// fixed-order reduction (deterministic, unlike the reference's atomics).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "phs_internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+namespace {
+
+constexpr int kTM = 128;  // time rows per workgroup
+constexpr int kTE = 64;   // electrodes per workgroup
+constexpr int kKS = 32;   // K per MFMA step
+
+__device__ __forceinline__ unsigned short bf16_rn(float x) {
+  unsigned int b = __float_as_uint(x);
+  b += 0x7FFFu + ((b >> 16) & 1u);
+  return (unsigned short)(b >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(unsigned short h) {
+  return __uint_as_float(((unsigned int)h) << 16);
+}
+
+__device__ __forceinline__ double w_axis(const double* __restrict__ vox, const double* __restrict__ el,
+                                         const double* __restrict__ c, const double* __restrict__ D,
+                                         int N, int v, int e, int k) {
+  const double lx = vox[3 * (size_t)v] - el[3 * e];
+  const double ly = vox[3 * (size_t)v + 1] - el[3 * e + 1];
+  const double lz = vox[3 * (size_t)v + 2] - el[3 * e + 2];
+  double l = sqrt(lx * lx + ly * ly + lz * lz);
+  if (l < 1.0) l = 1.0;
+  double proj;
+  if (D) {
+    // (D^T l)_k = sum_j l_j D_jk, D stored [9][N] row-major (D11, D12, D13, D21, ...)
+    proj = lx * D[(size_t)(0 + k) * N + v] + ly * D[(size_t)(3 + k) * N + v] +
+           lz * D[(size_t)(6 + k) * N + v];
+  } else {
+    proj = k == 0 ? lx : (k == 1 ? ly : lz);
+  }
+  return c[v] / (l * l * l) * proj;
+}
+
+__global__ void build_leadfield(int N, int E, int Kpad, const double* __restrict__ vox,
+                                const double* __restrict__ el, const double* __restrict__ c,
+                                const double* __restrict__ D, const int* __restrict__ nb,
+                                const int* __restrict__ perm, double inv2d, float* __restrict__ Wf,
+                                unsigned short* __restrict__ Whi, unsigned short* __restrict__ Wlo) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.y;
+  if (i >= Kpad) return;
+  const size_t o = (size_t)e * Kpad + i;
+  double acc = 0.0;
+  if (i < N && e < E) {
+    const int w = perm ? perm[i] : i;
+    for (int k = 0; k < 3; ++k) {
+      const int m = nb[6 * (size_t)w + 2 * k + 1];
+      if (m >= 0 && nb[6 * (size_t)m + 2 * k] >= 0 && nb[6 * (size_t)m + 2 * k + 1] >= 0)
+        acc += w_axis(vox, el, c, D, N, m, e, k);
+      const int p = nb[6 * (size_t)w + 2 * k];
+      if (p >= 0 && nb[6 * (size_t)p + 2 * k] >= 0 && nb[6 * (size_t)p + 2 * k + 1] >= 0)
+        acc -= w_axis(vox, el, c, D, N, p, e, k);
+    }
+    acc *= inv2d;
+  }
+  const float f = (float)acc;
+  const unsigned short hi = bf16_rn(f);
+  Wf[o] = f;
+  Whi[o] = hi;
+  Wlo[o] = bf16_rn(f - bf16_to_f(hi));
+}
+
+template <int PREC>  // 0: bf16 hi/lo (3 MFMA), 1: fp32 MFMA, 2: bf16 single
+__global__ void __launch_bounds__(256) egm_gemm(const float* __restrict__ S, int64_t T,
+                                                int64_t stride, int N, int Kpad, int k_chunk,
+                                                const float* __restrict__ Wf,
+                                                const unsigned short* __restrict__ Whi,
+                                                const unsigned short* __restrict__ Wlo, int Epad,
+                                                float* __restrict__ ws) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wt = wave & 1, we = wave >> 1;
+  const int64_t t0 = (int64_t)blockIdx.x * kTM + 64 * wt;
+  const int e0 = blockIdx.y * kTE + 32 * we;
+  const int split = blockIdx.z;
+  const int kb = split * k_chunk;
+  const int ke = min(N, kb + k_chunk);
+  const int q = lane >> 4;   // k sub-block of 8
+  const int r16 = lane & 15;
+
+  f32x4 acc[4][2];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) acc[r][cc] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const float* arow[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    int64_t t = t0 + 16 * r + r16;
+    if (t > T - 1) t = T - 1;
+    arow[r] = S + t * stride;
+  }
+  for (int kk = kb; kk < ke; kk += kKS) {
+    const int kl = kk + 8 * q;
+    float a[4][8];
+    const bool full = kk + kKS <= N;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a[r][j] = (full || kl + j < N) ? arow[r][kl + j] : 0.0f;
+    }
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) {
+      const size_t wo = (size_t)(e0 + 16 * cc + r16) * Kpad + kl;
+      if (PREC == 1) {
+        const float4 b0 = *(const float4*)(Wf + wo);
+        const float4 b1 = *(const float4*)(Wf + wo + 4);
+        const float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            acc[r][cc] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][j], bv[j], acc[r][cc], 0, 0, 0);
+      } else {
+        const u16x8 bh = *(const u16x8*)(Whi + wo);
+        const bf16x8 bhi = __builtin_bit_cast(bf16x8, bh);
+        bf16x8 blo;
+        if (PREC == 0) blo = __builtin_bit_cast(bf16x8, *(const u16x8*)(Wlo + wo));
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          u16x8 ah, al;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const unsigned short h = bf16_rn(a[r][j]);
+            ah[j] = h;
+            al[j] = bf16_rn(a[r][j] - bf16_to_f(h));
+          }
+          const bf16x8 ahi = __builtin_bit_cast(bf16x8, ah);
+          acc[r][cc] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, acc[r][cc], 0, 0, 0);
+          if (PREC == 0) {
+            const bf16x8 alo = __builtin_bit_cast(bf16x8, al);
+            acc[r][cc] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo, acc[r][cc], 0, 0, 0);
+            acc[r][cc] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi, acc[r][cc], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  // D tile layout: lane holds column (electrode) r16, rows (time) 4*q + j.
+  float* wsp = ws + (size_t)split * T * Epad;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t t = t0 + 16 * r + 4 * q + j;
+        if (t < T) wsp[t * Epad + e0 + 16 * cc + r16] = acc[r][cc][j];
+      }
+}
+
+__global__ void egm_reduce(const float* __restrict__ ws, int splits, int64_t T, int Epad, int E,
+                           float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= T * E) return;
+  const int64_t t = i / E;
+  const int e = (int)(i % E);
+  float s = 0.0f;
+  for (int k = 0; k < splits; ++k) s += ws[((size_t)k * T + t) * Epad + e];
+  out[i] = s;
+}
+
+__global__ void egm_ptp(const float* __restrict__ egm, int64_t T, int E, float* __restrict__ ptp) {
+  const int e = blockIdx.x;
+  float lo = INFINITY, hi = -INFINITY;
+  for (int64_t t = threadIdx.x; t < T; t += blockDim.x) {
+    const float v = egm[t * E + e];
+    lo = fminf(lo, v);
+    hi = fmaxf(hi, v);
+  }
+  __shared__ float slo[256], shi[256];
+  slo[threadIdx.x] = lo;
+  shi[threadIdx.x] = hi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      slo[threadIdx.x] = fminf(slo[threadIdx.x], slo[threadIdx.x + s]);
+      shi[threadIdx.x] = fmaxf(shi[threadIdx.x], shi[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ptp[e] = shi[0] - slo[0];
+}
+
+__global__ void egm_scale(float* __restrict__ egm, int64_t n, const float* __restrict__ ptp, int E,
+                          double typical) {
+  __shared__ double scale;
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int e = 0; e < E; ++e) m += ptp[e];
+    m /= E;
+    scale = typical / m;
+  }
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    egm[i] = (float)(egm[i] * scale);
+}
+
+}  // namespace
+
+struct phs_egm {
+  int N = 0, E = 0, Kpad = 0, Epad = 0;
+  float* Wf = nullptr;
+  unsigned short* Whi = nullptr;
+  unsigned short* Wlo = nullptr;
+  float* ws = nullptr;
+  size_t ws_bytes = 0;
+};
+
+extern "C" {
+
+int phs_egm_create(int32_t N, int32_t E, const double* voxel, const double* electrodes,
+                   const double* c_voxel, const double* D, const int32_t* nb, int32_t nb_stride,
+                   double delta, const int32_t* perm, phs_egm** handle, void* stream) {
+  if (N <= 0 || E <= 0 || !voxel || !electrodes || !c_voxel || !nb || nb_stride < 6 || !handle ||
+      !(delta > 0))
+    return phs::fail(PHS_ERR_ARG, "invalid electrogram arguments");
+  // The stencil transpose needs a symmetric face table: p_k(m_k(w)) == w.
+  std::vector<int> nb6((size_t)6 * N);
+  for (int32_t w = 0; w < N; ++w)
+    for (int k = 0; k < 6; ++k) {
+      const int32_t m = nb[(int64_t)w * nb_stride + k];
+      if (m < -1 || m >= N) return phs::fail(PHS_ERR_ARG, "neighbour id out of range");
+      nb6[(size_t)6 * w + k] = m;
+    }
+  for (int32_t w = 0; w < N; ++w)
+    for (int k = 0; k < 6; ++k) {
+      const int32_t m = nb6[(size_t)6 * w + k];
+      if (m >= 0 && nb6[(size_t)6 * m + (k ^ 1)] != w)
+        return phs::fail(PHS_ERR_UNSUPPORTED, "neighbour table is not symmetric");
+    }
+  if (perm) {
+    std::vector<char> seen(N, 0);
+    for (int32_t i = 0; i < N; ++i) {
+      if (perm[i] < 0 || perm[i] >= N || seen[perm[i]])
+        return phs::fail(PHS_ERR_ARG, "perm is not a permutation");
+      seen[perm[i]] = 1;
+    }
+  }
+  hipStream_t s = (hipStream_t)stream;
+  phs_egm* g = new phs_egm();
+  g->N = N;
+  g->E = E;
+  g->Kpad = (N + 63) / 64 * 64;
+  g->Epad = (E + kTE - 1) / kTE * kTE;
+  const size_t nw = (size_t)g->Epad * g->Kpad;
+  double *dv = nullptr, *de = nullptr, *dc = nullptr, *dD = nullptr;
+  int *dnb = nullptr, *dperm = nullptr;
+  hipError_t e = hipSuccess;
+  auto cleanup = [&](int code, const std::string& msg) {
+    hipFree(dv);
+    hipFree(de);
+    hipFree(dc);
+    hipFree(dD);
+    hipFree(dnb);
+    hipFree(dperm);
+    if (code) phs_egm_destroy(g);
+    return code ? phs::fail(code, msg) : PHS_OK;
+  };
+  if ((e = hipMalloc(&g->Wf, 4 * nw)) || (e = hipMalloc(&g->Whi, 2 * nw)) ||
+      (e = hipMalloc(&g->Wlo, 2 * nw)) || (e = hipMalloc(&dv, 24 * (size_t)N)) ||
+      (e = hipMalloc(&de, 24 * (size_t)E)) || (e = hipMalloc(&dc, 8 * (size_t)N)) ||
+      (D && (e = hipMalloc(&dD, 72 * (size_t)N))) || (e = hipMalloc(&dnb, 24 * (size_t)N)) ||
+      (perm && (e = hipMalloc(&dperm, 4 * (size_t)N))))
+    return cleanup(PHS_ERR_OOM, std::string("egm hipMalloc: ") + hipGetErrorString(e));
+  if ((e = hipMemcpyAsync(dv, voxel, 24 * (size_t)N, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(de, electrodes, 24 * (size_t)E, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(dc, c_voxel, 8 * (size_t)N, hipMemcpyHostToDevice, s)) ||
+      (D && (e = hipMemcpyAsync(dD, D, 72 * (size_t)N, hipMemcpyHostToDevice, s))) ||
+      (e = hipMemcpyAsync(dnb, nb6.data(), 24 * (size_t)N, hipMemcpyHostToDevice, s)) ||
+      (perm && (e = hipMemcpyAsync(dperm, perm, 4 * (size_t)N, hipMemcpyHostToDevice, s))))
+    return cleanup(PHS_ERR_HIP, std::string("egm upload: ") + hipGetErrorString(e));
+  const dim3 grid((g->Kpad + 255) / 256, g->Epad);
+  build_leadfield<<<grid, 256, 0, s>>>(N, E, g->Kpad, dv, de, dc, dD, dnb, dperm,
+                                       1.0 / (2.0 * delta), g->Wf, g->Whi, g->Wlo);
+  if ((e = hipGetLastError()) || (e = hipStreamSynchronize(s)))
+    return cleanup(PHS_ERR_HIP, std::string("build_leadfield: ") + hipGetErrorString(e));
+  *handle = g;
+  return cleanup(PHS_OK, "");
+}
+
+int phs_egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, float* egm,
+                int32_t precision, void* stream) {
+  if (!g || !samples || !egm || T <= 0 || stride < g->N)
+    return phs::fail(PHS_ERR_ARG, "invalid electrogram run arguments");
+  if (precision < 0 || precision > 2) return phs::fail(PHS_ERR_ARG, "precision must be 0, 1 or 2");
+  hipStream_t s = (hipStream_t)stream;
+  const int mt = (int)((T + kTM - 1) / kTM);
+  const int nt = g->Epad / kTE;
+  const int ksteps = (g->N + kKS - 1) / kKS;
+  int splits = std::max(1, std::min(ksteps, (2048 + mt * nt - 1) / (mt * nt)));
+  const int k_chunk = (ksteps + splits - 1) / splits * kKS;
+  splits = (g->N + k_chunk - 1) / k_chunk;
+  const size_t need = sizeof(float) * (size_t)splits * T * g->Epad;
+  if (need > g->ws_bytes) {
+    hipFree(g->ws);
+    g->ws = nullptr;
+    g->ws_bytes = 0;
+    PHS_TRY_ALLOC(g->ws, need);
+    g->ws_bytes = need;
+  }
+  const dim3 grid(mt, nt, splits);
+  if (precision == 0)
+    egm_gemm<0><<<grid, 256, 0, s>>>(samples, T, stride, g->N, g->Kpad, k_chunk, g->Wf, g->Whi,
+                                     g->Wlo, g->Epad, g->ws);
+  else if (precision == 1)
+    egm_gemm<1><<<grid, 256, 0, s>>>(samples, T, stride, g->N, g->Kpad, k_chunk, g->Wf, g->Whi,
+                                     g->Wlo, g->Epad, g->ws);
+  else
+    egm_gemm<2><<<grid, 256, 0, s>>>(samples, T, stride, g->N, g->Kpad, k_chunk, g->Wf, g->Whi,
+                                     g->Wlo, g->Epad, g->ws);
+  const int64_t n = T * g->E;
+  egm_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g->ws, splits, T, g->Epad, g->E, egm);
+  PHS_HIP_CHECK(hipGetLastError());
+  return PHS_OK;
+}
+
+int phs_egm_destroy(phs_egm* g) {
+  if (!g) return PHS_OK;
+  hipFree(g->Wf);
+  hipFree(g->Whi);
+  hipFree(g->Wlo);
+  hipFree(g->ws);
+  delete g;
+  return PHS_OK;
+}
+
+int phs_egm_normalize(float* egm, int64_t T, int32_t E, double typical, void* stream) {
+  if (!egm || T <= 0 || E <= 0) return phs::fail(PHS_ERR_ARG, "invalid normalize arguments");
+  hipStream_t s = (hipStream_t)stream;
+  float* ptp = nullptr;
+  PHS_TRY_ALLOC(ptp, sizeof(float) * E);
+  egm_ptp<<<E, 256, 0, s>>>(egm, T, E, ptp);
+  egm_scale<<<256, 256, 0, s>>>(egm, T * E, ptp, E, typical);
+  hipError_t e = hipGetLastError();
+  hipError_t e2 = hipStreamSynchronize(s);
+  hipFree(ptp);
+  if (e != hipSuccess || e2 != hipSuccess)
+    return phs::fail(PHS_ERR_HIP, std::string("normalize: ") + hipGetErrorString(e ? e : e2));
+  return PHS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,47 @@
+// Internal helpers shared by the libphs translation units (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/phs.h"
+
+namespace phs {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+// Chebyshev semi-iteration for A d = r with spec(A) in [lmin, lmax]:
+//   d_1 = r / theta,  d_{k+1} = a_k d_k + b_k d_{k-1} + g_k (r - A d_k)   (k >= 1)
+// with rho_0 = delta/theta, rho_k = 1 / (2 theta/delta - rho_{k-1}),
+//   a_k = 1 + rho_k rho_{k-1},  b_k = -rho_k rho_{k-1},  g_k = 2 rho_k / delta.
+struct ChebCoef {
+  double a, b, g;
+};
+void chebyshev_coefficients(double lmin, double lmax, int n, ChebCoef* out /* [n], out[0].g = 1/theta */);
+// Sweeps needed so that the Chebyshev error bound 2 q^m / (1 + q^2m) <= tol.
+int chebyshev_sweeps(double lmin, double lmax, double tol);
+
+}  // namespace phs
+
+#define PHS_HIP_CHECK(expr)                                                              \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      return phs::fail(PHS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define PHS_TRY_ALLOC(ptr, bytes)                                                       \
+  do {                                                                                  \
+    hipError_t _e = hipMalloc((void**)&(ptr), (bytes) > 0 ? (bytes) : 1);               \
+    if (_e != hipSuccess)                                                               \
+      return phs::fail(PHS_ERR_OOM, std::string("hipMalloc ") + std::to_string(bytes) + \
+                                        " bytes: " + hipGetErrorString(_e));            \
+  } while (0)
+
+// LAT bookkeeping in the per-voxel flag byte: bits 4-6 beat count, bit 7 above threshold.
+#define PHS_VF_BEAT_SHIFT 4
+#define PHS_VF_BEAT_MASK 0x70
+#define PHS_VF_ABOVE 0x80

@@ -1,0 +1,1009 @@
+// Bricked FP32 arena: the chosen form of the simulation hot path.
+//
+// Many simulations of uneven voxel counts share one set of device arrays; every launch
+// covers all resident bricks and each wavefront finds its simulation through the brick's
+// slot id.  Per step (simulation/simulation_gpu.py:358-407):
+//   reaction   RK4 in FP32, parameters as wave-uniform scalars, J from the per-step
+//              schedule byte and the voxel's site bits (replaces rk4_reaction_kernel
+//              simulation_gpu.py:71-130 and the per-step J_stim upload :373-377)
+//   rhs        r0 = dt L u* = 2 alpha sum_m (u*_m - u*_n), formed in FP64 from FP32 u*
+//   sweeps     Chebyshev semi-iteration on A d = r0, A = I - dt/2 L, FP32; the last sweep
+//              writes x = u* + d, the activation-time bookkeeping and the 1 kHz samples
+//              (replaces build_diffusion_matrix_gpu :138-225 and the GMRES CN solve
+//              :227-259, :388, and the sampling copies :391-407)
+//   refine     optional FP64-residual refinement: r = r0 - A d in FP64, Chebyshev on the
+//              correction in FP32, accumulation in FP64
+//   advance    per-simulation step counters (device side, so steps can be graph-replayed)
+// A brick is 64 lattice cells handled by one wavefront; in-brick neighbour values move
+// between lanes with ds_bpermute (__shfl), cross-brick face neighbours are gathered from
+// the neighbour brick's compact range.  A missing neighbour contributes u_m - u_n = 0
+// (zero-flux mirror).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "phs_internal.h"
+#include "plan.h"
+
+namespace {
+
+constexpr int kWaves = 4;
+constexpr int kBlock = 64 * kWaves;
+constexpr int kMaxSweeps = 64;
+
+struct BrickRec {
+  uint64_t mask;
+  int32_t start;
+  int32_t slot;
+  int32_t nbb[6];
+  int32_t pad[2];
+};
+static_assert(sizeof(BrickRec) == 48, "brick record layout");
+
+struct SlotRec {
+  int64_t step;
+  int64_t n_steps;
+  int64_t voxel_off;
+  const uint8_t* sched;
+  float* su;
+  float* sh;
+  int32_t* lat;
+  int32_t n_voxel;
+  int32_t active;
+  int32_t model;
+  int32_t nspm;
+  int32_t n_samples;
+  int32_t max_beats;
+  int32_t needs_vf;  // scar or temporary-block voxels present
+  int32_t hetero;    // per-voxel alpha in the ALPHA array
+  float p[5];        // MS: 1/tau_open, 1/tau_close, 1/tau_in, 1/tau_out, v_gate; AP: k,a,eps0,mu1,mu2
+  float dt;
+  float alpha;
+  float lat_thr;
+  double alpha64;
+  float coef[kMaxSweeps][4];  // a, b, g, unused; coef[0][2] = 1/theta
+};
+
+struct ArenaDev {
+  const BrickRec* __restrict__ bricks;
+  SlotRec* slots;
+  float* U;
+  float* H;
+  float* R0;
+  double* DACC;
+  float* USTAR;
+  uint8_t* VF;
+  const float* ALPHA;
+  int32_t n_bricks;
+  int32_t xcd;
+  int32_t diag;
+};
+
+enum { MODE_MID = 0, MODE_FINAL_X = 1, MODE_DACC_SET = 2, MODE_DACC_ADD = 3, MODE_DACC_ADD_X = 4 };
+
+__device__ __forceinline__ int wave_brick(int xcd) {
+  int blk = blockIdx.x;
+  if (xcd) {
+    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous
+    // run of bricks so halo gathers hit that XCD's L2.  The grid is a multiple of 8.
+    const int per = gridDim.x >> 3;
+    blk = (blk & 7) * per + (blk >> 3);
+  }
+  return __builtin_amdgcn_readfirstlane(blk * kWaves + (int)(threadIdx.x >> 6));
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ int compact_index(const BrickRec& br, int lane) {
+  return br.start + __popcll(br.mask & ((1ULL << lane) - 1ULL));
+}
+
+__device__ __forceinline__ uint8_t sched_now(const SlotRec& S) { return S.sched[S.step]; }
+
+// Effective alpha = dt c / (2 Delta^2) of voxel idx for this step.
+template <typename T>
+__device__ __forceinline__ T alpha_of(const ArenaDev& A, const SlotRec& S, int idx, uint8_t sc,
+                                      bool valid) {
+  T a = sizeof(T) == 8 ? (T)S.alpha64 : (T)S.alpha;
+  if (S.hetero && valid) a = (T)A.ALPHA[idx];
+  if (S.needs_vf && valid) {
+    const uint8_t f = A.VF[idx];
+    if ((f & PHS_VF_SCAR) || ((sc & PHS_SCHED_TEMP_BLOCK) && (f & PHS_VF_TEMP_BLOCK))) a = (T)0;
+  }
+  return a;
+}
+
+// sum over present face neighbours m of (v_m - v_n), v held per lane (0 on empty lanes),
+// cross-brick values gathered from src[] (scaled by sc).  Accumulated in type T.
+template <int BX, int BY, int BZ, typename T, typename V>
+__device__ __forceinline__ T neighbour_sum(const ArenaDev& A, const BrickRec& br, int lane,
+                                           bool valid, V v, const V* __restrict__ src, V sc) {
+  constexpr int B[3] = {BX, BY, BZ};
+  constexpr int St[3] = {1, BX, BX * BY};
+  const int l[3] = {lane % BX, (lane / BX) % BY, lane / (BX * BY)};
+  T s = (T)0;
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    const int ax = d >> 1;
+    const int sg = (d & 1) ? -1 : 1;
+    const bool inb = sg > 0 ? (l[ax] < B[ax] - 1) : (l[ax] > 0);
+    const int partner = inb ? lane + sg * St[ax] : lane;
+    const V pv = __shfl(v, partner);
+    const int nbk = __builtin_amdgcn_readfirstlane(br.nbb[d]);
+    uint64_t nm = 0;
+    int ns = 0;
+    if (nbk >= 0) {
+      nm = A.bricks[nbk].mask;
+      ns = A.bricks[nbk].start;
+    }
+    if (inb) {
+      if ((br.mask >> partner) & 1ULL) s += (T)pv - (T)v;
+    } else if (valid && nbk >= 0) {
+      const int cell = lane - sg * (B[ax] - 1) * St[ax];
+      if ((nm >> cell) & 1ULL) {
+        const V w = src[ns + __popcll(nm & ((1ULL << cell) - 1ULL))] * sc;
+        s += (T)w - (T)v;
+      }
+    }
+  }
+  return s;
+}
+
+__device__ __forceinline__ void rhs32(const SlotRec& S, float u, float h, float J, float& du,
+                                      float& dh) {
+  if (S.model == 0) {
+    du = h * u * u * (1.0f - u) * S.p[2] - u * S.p[3] + J;
+    dh = (u < S.p[4]) ? (1.0f - h) * S.p[0] : -h * S.p[1];
+  } else {
+    const float k = S.p[0], a = S.p[1];
+    du = -k * u * (u - a) * (u - 1.0f) - u * h + J;
+    dh = (S.p[2] + S.p[3] * h * __builtin_amdgcn_rcpf(u + S.p[4])) * (-h - k * u * (u - a - 1.0f));
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) arena_reaction(ArenaDev A) {
+  const int b = wave_brick(A.xcd);
+  if (b >= A.n_bricks) return;
+  const BrickRec& br = A.bricks[b];
+  const int slot = br.slot;
+  if (slot < 0) return;
+  const SlotRec& S = A.slots[slot];
+  if (!S.active || S.step >= S.n_steps) return;
+  const int lane = lane_id();
+  if (!((br.mask >> lane) & 1ULL)) return;
+  const int idx = compact_index(br, lane);
+  const uint8_t sc = sched_now(S);
+  const uint8_t f = A.VF[idx];
+  const float J = (((sc & PHS_SCHED_S1) && (f & PHS_VF_S1)) || ((sc & PHS_SCHED_S2) && (f & PHS_VF_S2)))
+                      ? 1.0f
+                      : 0.0f;
+  const float dt = S.dt;
+  const float u = A.U[idx], h = A.H[idx];
+  float k1u, k1h, k2u, k2h, k3u, k3h, k4u, k4h;
+  rhs32(S, u, h, J, k1u, k1h);
+  rhs32(S, u + 0.5f * dt * k1u, h + 0.5f * dt * k1h, J, k2u, k2h);
+  rhs32(S, u + 0.5f * dt * k2u, h + 0.5f * dt * k2h, J, k3u, k3h);
+  rhs32(S, u + dt * k3u, h + dt * k3h, J, k4u, k4h);
+  const float s6 = dt * (1.0f / 6.0f);
+  A.U[idx] = u + s6 * (k1u + 2.0f * k2u + 2.0f * k3u + k4u);
+  A.H[idx] = h + s6 * (k1h + 2.0f * k2h + 2.0f * k3h + k4h);
+}
+
+// r0 = 2 alpha sum (u*_m - u*_n), in FP64 from FP32 u*.
+template <int BX, int BY, int BZ>
+__global__ void __launch_bounds__(kBlock) arena_rhs(ArenaDev A) {
+  const int b = wave_brick(A.xcd);
+  if (b >= A.n_bricks) return;
+  const BrickRec& br = A.bricks[b];
+  const int slot = br.slot;
+  if (slot < 0) return;
+  const SlotRec& S = A.slots[slot];
+  if (!S.active || S.step >= S.n_steps) return;
+  const int lane = lane_id();
+  const bool valid = (br.mask >> lane) & 1ULL;
+  const int idx = compact_index(br, lane);
+  const float un = valid ? A.U[idx] : 0.0f;
+  const double s = neighbour_sum<BX, BY, BZ, double, float>(A, br, lane, valid, un, A.U, 1.0f);
+  if (!valid) return;
+  const double a = alpha_of<double>(A, S, idx, sched_now(S), valid);
+  A.R0[idx] = (float)(2.0 * a * s);
+}
+
+// FP64 residual of the accumulated correction: r = 2 alpha sum(u*_m - u*_n) - A dacc.
+template <int BX, int BY, int BZ>
+__global__ void __launch_bounds__(kBlock) arena_residual(ArenaDev A) {
+  const int b = wave_brick(A.xcd);
+  if (b >= A.n_bricks) return;
+  const BrickRec& br = A.bricks[b];
+  const int slot = br.slot;
+  if (slot < 0) return;
+  const SlotRec& S = A.slots[slot];
+  if (!S.active || S.step >= S.n_steps) return;
+  const int lane = lane_id();
+  const bool valid = (br.mask >> lane) & 1ULL;
+  const int idx = compact_index(br, lane);
+  const float un = valid ? A.U[idx] : 0.0f;
+  const double su = neighbour_sum<BX, BY, BZ, double, float>(A, br, lane, valid, un, A.U, 1.0f);
+  const double dn = valid ? A.DACC[idx] : 0.0;
+  const double sd = neighbour_sum<BX, BY, BZ, double, double>(A, br, lane, valid, dn, A.DACC, 1.0);
+  if (!valid) return;
+  const double a = alpha_of<double>(A, S, idx, sched_now(S), valid);
+  A.R0[idx] = (float)(2.0 * a * su - (dn - a * sd));
+}
+
+// Activation time, samples, and state write of the final pass.
+__device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotRec& S, int idx, float x) {
+  A.U[idx] = x;
+  const uint8_t f = A.VF[idx];
+  const bool now = x >= S.lat_thr;
+  const bool above = (f & PHS_VF_ABOVE) != 0;
+  const int64_t local = (int64_t)idx - S.voxel_off;
+  if (now && !above) {
+    int bt = (f & PHS_VF_BEAT_MASK) >> PHS_VF_BEAT_SHIFT;
+    if (bt < S.max_beats && S.lat) S.lat[(int64_t)bt * S.n_voxel + local] = (int32_t)S.step;
+    bt = bt < 7 ? bt + 1 : 7;
+    A.VF[idx] = (uint8_t)((f & 0x0F) | (bt << PHS_VF_BEAT_SHIFT) | PHS_VF_ABOVE);
+  } else if (!now && above) {
+    A.VF[idx] = (uint8_t)(f & ~PHS_VF_ABOVE);
+  }
+  int64_t row = -1;
+  if (S.nspm > 0) {
+    if (S.step % S.nspm == 0) row = S.step / S.nspm;
+  } else {
+    row = S.step;
+  }
+  if (row >= 0 && row < S.n_samples) {
+    const int64_t o = row * S.n_voxel + local;
+    if (S.su) S.su[o] = x;
+    if (S.sh) S.sh[o] = A.H[idx];
+  }
+}
+
+// One Chebyshev sweep d_{k+1} = a_k d_k + b_k d_{k-1} + g_k (r0 - A d_k).
+// cur_kind 1: d_k = r0 / theta (k = 1).  prev_kind 0: d_{k-1} = 0; 1: r0 / theta; 2: prev[].
+template <int BX, int BY, int BZ, int MODE>
+__global__ void __launch_bounds__(kBlock) arena_sweep(ArenaDev A, int k, const float* cur,
+                                                      int cur_kind, const float* prev,
+                                                      int prev_kind, float* out) {
+  const int b = wave_brick(A.xcd);
+  if (b >= A.n_bricks) return;
+  const BrickRec& br = A.bricks[b];
+  const int slot = br.slot;
+  if (slot < 0) return;
+  const SlotRec& S = A.slots[slot];
+  if (!S.active || S.step >= S.n_steps) return;
+  const int lane = lane_id();
+  const bool valid = (br.mask >> lane) & 1ULL;
+  const int idx = compact_index(br, lane);
+  const float inv_theta = S.coef[0][2];
+  const float* src = cur_kind ? A.R0 : cur;
+  const float sc = cur_kind ? inv_theta : 1.0f;
+  const float dn = valid ? src[idx] * sc : 0.0f;
+  const float s = neighbour_sum<BX, BY, BZ, float, float>(A, br, lane, valid, dn, src, sc);
+  if (!valid) return;
+  const uint8_t schedb = sched_now(S);
+  const float a = alpha_of<float>(A, S, idx, schedb, valid);
+  const float r0 = A.R0[idx];
+  const float Ad = dn - a * s;
+  const float dp = prev_kind == 0 ? 0.0f : (prev_kind == 1 ? r0 * inv_theta : prev[idx]);
+  const float dnew = S.coef[k][0] * dn + S.coef[k][1] * dp + S.coef[k][2] * (r0 - Ad);
+  if (MODE == MODE_MID) {
+    out[idx] = dnew;
+  } else if (MODE == MODE_FINAL_X) {
+    const float us = A.U[idx];
+    if (A.diag) {
+      A.USTAR[idx] = us;
+      A.DACC[idx] = (double)dnew;
+    }
+    finish_voxel(A, S, idx, us + dnew);
+  } else if (MODE == MODE_DACC_SET) {
+    A.DACC[idx] = (double)dnew;
+  } else if (MODE == MODE_DACC_ADD) {
+    A.DACC[idx] += (double)dnew;
+  } else {
+    const double acc = A.DACC[idx] + (double)dnew;
+    const float us = A.U[idx];
+    A.DACC[idx] = acc;
+    if (A.diag) A.USTAR[idx] = us;
+    finish_voxel(A, S, idx, (float)((double)us + acc));
+  }
+}
+
+__global__ void arena_advance(SlotRec* slots, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n && slots[s].active && slots[s].step < slots[s].n_steps) slots[s].step += 1;
+}
+
+__global__ void arena_deactivate(SlotRec* slots, int slot) { slots[slot].active = 0; }
+
+__global__ void arena_invalidate(BrickRec* bricks, int64_t b0, int64_t nb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nb) bricks[b0 + i].slot = -1;
+}
+
+// Diagnostic: sum of r^2 and b^2 for b = u* + alpha sum(u*_m - u*_n),
+// x = u* + dacc, r = b - (x - alpha sum(x_m - x_n)), at the last completed step.
+template <int BX, int BY, int BZ>
+__global__ void __launch_bounds__(kBlock) arena_diag(ArenaDev A, int slot_sel, double* acc) {
+  const int b = wave_brick(A.xcd);
+  if (b >= A.n_bricks) return;
+  const BrickRec& br = A.bricks[b];
+  if (br.slot != slot_sel) return;
+  const SlotRec& S = A.slots[br.slot];
+  const int lane = lane_id();
+  const bool valid = (br.mask >> lane) & 1ULL;
+  const int idx = compact_index(br, lane);
+  const float us = valid ? A.USTAR[idx] : 0.0f;
+  const double su = neighbour_sum<BX, BY, BZ, double, float>(A, br, lane, valid, us, A.USTAR, 1.0f);
+  const double dn = valid ? A.DACC[idx] : 0.0;
+  const double sd = neighbour_sum<BX, BY, BZ, double, double>(A, br, lane, valid, dn, A.DACC, 1.0);
+  double r2 = 0.0, b2 = 0.0;
+  if (valid) {
+    const uint8_t sc = S.step > 0 ? S.sched[S.step - 1] : 0;
+    double a = S.hetero ? (double)A.ALPHA[idx] : S.alpha64;
+    if (S.needs_vf) {
+      const uint8_t f = A.VF[idx];
+      if ((f & PHS_VF_SCAR) || ((sc & PHS_SCHED_TEMP_BLOCK) && (f & PHS_VF_TEMP_BLOCK))) a = 0.0;
+    }
+    const double bb = (double)us + a * su;
+    // x = u* + d: A x = u* + d - alpha (su + sd)
+    const double r = bb - ((double)us + dn - a * (su + sd));
+    r2 = r * r;
+    b2 = bb * bb;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    r2 += __shfl_xor(r2, o);
+    b2 += __shfl_xor(b2, o);
+  }
+  if (lane == 0) {
+    atomicAdd(acc, r2);
+    atomicAdd(acc + 1, b2);
+  }
+}
+
+__global__ void scatter_rows_f32(float* __restrict__ dst, const float* __restrict__ src,
+                                 const int32_t* __restrict__ perm, int64_t rows, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = blockIdx.y;
+  for (int64_t rr = r; rr < rows; rr += gridDim.y)
+    if (i < n) dst[rr * n + perm[i]] = src[rr * n + i];
+}
+
+__global__ void scatter_rows_i32(int32_t* __restrict__ dst, const int32_t* __restrict__ src,
+                                 const int32_t* __restrict__ perm, int64_t rows, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t rr = blockIdx.y; rr < rows; rr += gridDim.y)
+    if (i < n) dst[rr * n + perm[i]] = src[rr * n + i];
+}
+
+// ------------------------------------------------------------------ host side
+class RangeAlloc {
+ public:
+  void init(int64_t cap) {
+    free_.clear();
+    if (cap > 0) free_[0] = cap;
+  }
+  int64_t alloc(int64_t n) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        const int64_t at = it->first;
+        const int64_t rem = it->second - n;
+        free_.erase(it);
+        if (rem > 0) free_[at + n] = rem;
+        return at;
+      }
+    }
+    return -1;
+  }
+  void release(int64_t at, int64_t n) {
+    auto it = free_.emplace(at, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+
+ private:
+  std::map<int64_t, int64_t> free_;
+};
+
+struct SlotHost {
+  bool used = false;
+  int64_t voxel_off = 0, brick_off = 0;
+  int32_t n_voxel = 0, n_bricks = 0;
+  int64_t n_steps = 0, steps_done = 0;
+  std::vector<int32_t> perm;
+  int32_t* perm_dev = nullptr;
+  uint8_t* sched_dev = nullptr;
+  double lmin = 1.0, lmax = 1.0;
+  void* staging = nullptr;  // pinned upload buffer, freed after its copies complete
+  hipEvent_t upload_done = nullptr;
+};
+
+}  // namespace
+
+struct DeferredFree {
+  hipEvent_t done = nullptr;
+  void* host = nullptr;
+  void* dev[2] = {nullptr, nullptr};
+};
+
+struct phs_arena {
+  phs_arena_config cfg{};
+  std::vector<DeferredFree> graveyard;
+  BrickRec* bricks = nullptr;
+  SlotRec* slots = nullptr;
+  float *U = nullptr, *H = nullptr, *R0 = nullptr, *DA = nullptr, *DB = nullptr,
+        *USTAR = nullptr, *ALPHA = nullptr;
+  double* DACC = nullptr;
+  uint8_t* VF = nullptr;
+  double* diag_acc = nullptr;
+  std::vector<SlotHost> sh;
+  RangeAlloc vox, brk;
+  int64_t brick_high = 0;
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int32_t g_steps = -1;
+  int64_t g_grid = -1;
+};
+
+namespace {
+
+ArenaDev dev_view(const phs_arena* a) {
+  ArenaDev A;
+  A.bricks = a->bricks;
+  A.slots = a->slots;
+  A.U = a->U;
+  A.H = a->H;
+  A.R0 = a->R0;
+  A.DACC = a->DACC;
+  A.USTAR = a->USTAR;
+  A.VF = a->VF;
+  A.ALPHA = a->ALPHA;
+  A.n_bricks = (int32_t)a->brick_high;
+  A.xcd = a->cfg.xcd_remap ? 1 : 0;
+  A.diag = a->cfg.diag ? 1 : 0;
+  return A;
+}
+
+int grid_for(const phs_arena* a) {
+  int g = (int)((a->brick_high + kWaves - 1) / kWaves);
+  if (g < 1) g = 1;
+  if (a->cfg.xcd_remap) g = (g + 7) / 8 * 8;
+  return g;
+}
+
+template <int BX, int BY, int BZ>
+void launch_step_shape(phs_arena* a, hipStream_t s) {
+  const ArenaDev A = dev_view(a);
+  const int grid = grid_for(a);
+  const phs_arena_config& c = a->cfg;
+  arena_reaction<<<grid, kBlock, 0, s>>>(A);
+  arena_rhs<BX, BY, BZ><<<grid, kBlock, 0, s>>>(A);
+  float* bufs[2] = {a->DA, a->DB};
+  // One Chebyshev solve of m sweeps on the right-hand side in R0; `last` selects the final
+  // pass behaviour.
+  auto solve = [&](int m, int last_mode) {
+    for (int k = 1; k < m; ++k) {
+      const float* cur = k == 1 ? nullptr : bufs[k % 2];
+      const int cur_kind = k == 1 ? 1 : 0;
+      const float* prev = k <= 2 ? nullptr : bufs[(k - 1) % 2];
+      const int prev_kind = k == 1 ? 0 : (k == 2 ? 1 : 2);
+      float* out = bufs[(k + 1) % 2];
+      if (k < m - 1) {
+        arena_sweep<BX, BY, BZ, MODE_MID><<<grid, kBlock, 0, s>>>(A, k, cur, cur_kind, prev,
+                                                                    prev_kind, out);
+      } else {
+        switch (last_mode) {
+          case MODE_FINAL_X:
+            arena_sweep<BX, BY, BZ, MODE_FINAL_X><<<grid, kBlock, 0, s>>>(A, k, cur, cur_kind, prev,
+                                                                            prev_kind, out);
+            break;
+          case MODE_DACC_SET:
+            arena_sweep<BX, BY, BZ, MODE_DACC_SET><<<grid, kBlock, 0, s>>>(A, k, cur, cur_kind,
+                                                                             prev, prev_kind, out);
+            break;
+          case MODE_DACC_ADD:
+            arena_sweep<BX, BY, BZ, MODE_DACC_ADD><<<grid, kBlock, 0, s>>>(A, k, cur, cur_kind,
+                                                                             prev, prev_kind, out);
+            break;
+          default:
+            arena_sweep<BX, BY, BZ, MODE_DACC_ADD_X><<<grid, kBlock, 0, s>>>(A, k, cur, cur_kind,
+                                                                               prev, prev_kind, out);
+        }
+      }
+    }
+  };
+  if (c.refine <= 0) {
+    solve(c.sweeps, MODE_FINAL_X);
+  } else {
+    solve(c.sweeps, MODE_DACC_SET);
+    for (int r = 0; r < c.refine; ++r) {
+      arena_residual<BX, BY, BZ><<<grid, kBlock, 0, s>>>(A);
+      solve(c.refine_sweeps, r == c.refine - 1 ? MODE_DACC_ADD_X : MODE_DACC_ADD);
+    }
+  }
+  arena_advance<<<(c.max_sims + 63) / 64, 64, 0, s>>>(a->slots, c.max_sims);
+}
+
+template <int BX, int BY, int BZ>
+void launch_diag_shape(phs_arena* a, int slot, hipStream_t s) {
+  arena_diag<BX, BY, BZ><<<grid_for(a), kBlock, 0, s>>>(dev_view(a), slot, a->diag_acc);
+}
+
+bool shape_supported(const int* b) {
+  const int s[][3] = {{4, 4, 4}, {8, 8, 1}, {8, 4, 2}, {4, 8, 2}, {16, 4, 1}};
+  for (auto& q : s)
+    if (q[0] == b[0] && q[1] == b[1] && q[2] == b[2]) return true;
+  return false;
+}
+
+void launch_step(phs_arena* a, hipStream_t s) {
+  const int* b = a->cfg.brick;
+  if (b[0] == 8 && b[1] == 8) launch_step_shape<8, 8, 1>(a, s);
+  else if (b[0] == 8 && b[1] == 4) launch_step_shape<8, 4, 2>(a, s);
+  else if (b[0] == 4 && b[1] == 8) launch_step_shape<4, 8, 2>(a, s);
+  else if (b[0] == 16) launch_step_shape<16, 4, 1>(a, s);
+  else launch_step_shape<4, 4, 4>(a, s);
+}
+
+void launch_diag(phs_arena* a, int slot, hipStream_t s) {
+  const int* b = a->cfg.brick;
+  if (b[0] == 8 && b[1] == 8) launch_diag_shape<8, 8, 1>(a, slot, s);
+  else if (b[0] == 8 && b[1] == 4) launch_diag_shape<8, 4, 2>(a, slot, s);
+  else if (b[0] == 4 && b[1] == 8) launch_diag_shape<4, 8, 2>(a, slot, s);
+  else if (b[0] == 16) launch_diag_shape<16, 4, 1>(a, slot, s);
+  else launch_diag_shape<4, 4, 4>(a, slot, s);
+}
+
+void free_slot_host(SlotHost& h) {
+  if (h.upload_done) {
+    hipEventSynchronize(h.upload_done);
+    hipEventDestroy(h.upload_done);
+    h.upload_done = nullptr;
+  }
+  if (h.staging) hipHostFree(h.staging);
+  h.staging = nullptr;
+  hipFree(h.perm_dev);
+  hipFree(h.sched_dev);
+  h.perm_dev = nullptr;
+  h.sched_dev = nullptr;
+}
+
+// Free slot resources whose last use (recorded by `done`) has completed; `wait` blocks.
+void reap(phs_arena* a, bool wait) {
+  std::vector<DeferredFree> keep;
+  for (auto& g : a->graveyard) {
+    if (g.done && !wait && hipEventQuery(g.done) == hipErrorNotReady) {
+      keep.push_back(g);
+      continue;
+    }
+    if (g.done) {
+      hipEventSynchronize(g.done);
+      hipEventDestroy(g.done);
+    }
+    if (g.host) hipHostFree(g.host);
+    hipFree(g.dev[0]);
+    hipFree(g.dev[1]);
+  }
+  a->graveyard.swap(keep);
+}
+
+void recompute_high(phs_arena* a) {
+  int64_t hi = 0;
+  for (auto& h : a->sh)
+    if (h.used) hi = std::max(hi, h.brick_off + h.n_bricks);
+  a->brick_high = hi;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phs_arena_create(const phs_arena_config* cfg, phs_arena** handle) {
+  if (!cfg || !handle) return phs::fail(PHS_ERR_ARG, "NULL argument");
+  if (cfg->max_voxels <= 0 || cfg->max_bricks <= 0 || cfg->max_sims <= 0)
+    return phs::fail(PHS_ERR_ARG, "capacities must be > 0");
+  if (cfg->max_voxels >= (1LL << 31) || cfg->max_bricks >= (1LL << 28))
+    return phs::fail(PHS_ERR_ARG, "capacity exceeds 32-bit compact indexing");
+  if (!shape_supported(cfg->brick))
+    return phs::fail(PHS_ERR_ARG, "brick shape must be one of 4x4x4, 8x8x1, 8x4x2, 4x8x2, 16x4x1");
+  if (cfg->sweeps < 2 || cfg->sweeps > kMaxSweeps)
+    return phs::fail(PHS_ERR_ARG, "sweeps must be in [2, 64]");
+  if (cfg->refine < 0 || cfg->refine > 4) return phs::fail(PHS_ERR_ARG, "refine must be in [0, 4]");
+  if (cfg->refine > 0 && (cfg->refine_sweeps < 2 || cfg->refine_sweeps > kMaxSweeps))
+    return phs::fail(PHS_ERR_ARG, "refine_sweeps must be in [2, 64]");
+  phs_arena* a = new phs_arena();
+  a->cfg = *cfg;
+  const size_t nv = (size_t)cfg->max_voxels;
+  hipError_t e = hipSuccess;
+  const bool need_dacc = cfg->refine > 0 || cfg->diag;
+  if ((e = hipMalloc(&a->bricks, sizeof(BrickRec) * (size_t)cfg->max_bricks)) ||
+      (e = hipMalloc(&a->slots, sizeof(SlotRec) * (size_t)cfg->max_sims)) ||
+      (e = hipMalloc(&a->U, 4 * nv)) || (e = hipMalloc(&a->H, 4 * nv)) ||
+      (e = hipMalloc(&a->R0, 4 * nv)) || (e = hipMalloc(&a->DA, 4 * nv)) ||
+      (e = hipMalloc(&a->DB, 4 * nv)) || (e = hipMalloc(&a->VF, nv)) ||
+      (e = hipMalloc(&a->ALPHA, 4 * nv)) || (e = hipMalloc(&a->diag_acc, 16)) ||
+      (need_dacc && (e = hipMalloc(&a->DACC, 8 * nv))) ||
+      (cfg->diag && (e = hipMalloc(&a->USTAR, 4 * nv)))) {
+    phs_arena_destroy(a);
+    return phs::fail(PHS_ERR_OOM, std::string("arena hipMalloc: ") + hipGetErrorString(e));
+  }
+  std::vector<SlotRec> zero(cfg->max_sims);
+  std::memset(zero.data(), 0, sizeof(SlotRec) * zero.size());
+  if ((e = hipMemcpy(a->slots, zero.data(), sizeof(SlotRec) * zero.size(), hipMemcpyHostToDevice)) ||
+      (e = hipMemset(a->bricks, 0xFF, sizeof(BrickRec) * (size_t)cfg->max_bricks))) {
+    phs_arena_destroy(a);
+    return phs::fail(PHS_ERR_HIP, std::string("arena init: ") + hipGetErrorString(e));
+  }
+  a->sh.resize(cfg->max_sims);
+  a->vox.init(cfg->max_voxels);
+  a->brk.init(cfg->max_bricks);
+  *handle = a;
+  return PHS_OK;
+}
+
+int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, int32_t* slot_out,
+                  void* stream) {
+  if (!a || !d || !slot_out) return phs::fail(PHS_ERR_ARG, "NULL argument");
+  if (d->n_voxel <= 0 || (d->model != 0 && d->model != 1) || !(d->dt > 0) || !(d->delta > 0))
+    return phs::fail(PHS_ERR_ARG, "invalid simulation description");
+  if (!d->ijk) return phs::fail(PHS_ERR_ARG, "lattice coordinates (ijk) required");
+  if (d->n_steps > 0 && !d->schedule) return phs::fail(PHS_ERR_ARG, "schedule required");
+  if (out && out->max_beats > 7) return phs::fail(PHS_ERR_ARG, "max_beats must be <= 7");
+  if (out && out->n_probes > 0) return phs::fail(PHS_ERR_ARG, "probes are a plain-form output");
+  if (d->nb && d->nb_stride < 6) return phs::fail(PHS_ERR_ARG, "nb_stride must be >= 6");
+  hipStream_t s = (hipStream_t)stream;
+  reap(a, false);
+  int slot = -1;
+  for (int i = 0; i < a->cfg.max_sims; ++i)
+    if (!a->sh[i].used) {
+      slot = i;
+      break;
+    }
+  if (slot < 0) return phs::fail(PHS_ERR_CAPACITY, "no free simulation slot");
+  phs::BrickPlan P;
+  std::string err;
+  const int rc = phs::build_plan(d->n_voxel, d->ijk, d->nb, d->nb_stride, a->cfg.brick[0],
+                                 a->cfg.brick[1], a->cfg.brick[2], P, err);
+  if (rc == -2) return phs::fail(PHS_ERR_UNSUPPORTED, err);
+  if (rc) return phs::fail(PHS_ERR_ARG, err);
+  const int64_t vo = a->vox.alloc(d->n_voxel);
+  if (vo < 0) return phs::fail(PHS_ERR_CAPACITY, "arena voxel capacity exhausted");
+  const int64_t bo = a->brk.alloc(P.n_bricks);
+  if (bo < 0) {
+    a->vox.release(vo, d->n_voxel);
+    return phs::fail(PHS_ERR_CAPACITY, "arena brick capacity exhausted");
+  }
+  SlotHost& h = a->sh[slot];
+  h = SlotHost();
+  h.voxel_off = vo;
+  h.brick_off = bo;
+  h.n_voxel = d->n_voxel;
+  h.n_bricks = P.n_bricks;
+  h.n_steps = d->n_steps;
+  h.perm = P.perm;
+  const int N = d->n_voxel;
+  auto undo = [&](int code, const std::string& msg) {
+    free_slot_host(h);
+    a->vox.release(vo, N);
+    a->brk.release(bo, P.n_bricks);
+    h = SlotHost();
+    return phs::fail(code, msg);
+  };
+
+  // Host staging (pinned): bricks | slot | U | H | VF | ALPHA | perm | schedule
+  const size_t sz_br = sizeof(BrickRec) * (size_t)P.n_bricks;
+  const size_t sz_sl = sizeof(SlotRec);
+  const size_t sz_f = 4 * (size_t)N;
+  const size_t total = sz_br + sz_sl + 4 * sz_f + (size_t)N + (size_t)d->n_steps + 64;
+  hipError_t e = hipHostMalloc(&h.staging, total, hipHostMallocDefault);
+  if (e != hipSuccess) return undo(PHS_ERR_OOM, "hipHostMalloc staging");
+  char* st = (char*)h.staging;
+  BrickRec* hb = (BrickRec*)st;
+  SlotRec* hs = (SlotRec*)(st + sz_br);
+  float* hu = (float*)(st + sz_br + sz_sl);
+  float* hh = hu + N;
+  float* hal = hh + N;
+  int32_t* hperm = (int32_t*)(hal + N);
+  uint8_t* hvf = (uint8_t*)(hperm + N);
+  uint8_t* hsched = hvf + N;
+
+  bool hetero = false;
+  bool needs_vf = false;
+  double amax_deg = 0.0;
+  const double a0 = d->dt * d->c / (2.0 * d->delta * d->delta);
+  for (int32_t b = 0; b < P.n_bricks; ++b) {
+    BrickRec r;
+    r.mask = P.mask[b];
+    r.start = (int32_t)(vo + P.start[b]);
+    r.slot = slot;
+    for (int k = 0; k < 6; ++k) {
+      const int32_t nb = P.nbb[(size_t)b * 6 + k];
+      r.nbb[k] = nb >= 0 ? (int32_t)(bo + nb) : -1;
+    }
+    r.pad[0] = r.pad[1] = 0;
+    hb[b] = r;
+  }
+  for (int32_t c = 0; c < N; ++c) {
+    const int32_t o = P.perm[c];
+    hperm[c] = o;
+    hu[c] = d->u0 ? d->u0[o] : 0.0f;
+    hh[c] = d->h0 ? d->h0[o] : (d->model == 0 ? 1.0f : 0.0f);
+    uint8_t f = d->vflags ? (uint8_t)(d->vflags[o] & 0x0F) : 0;
+    if (f & (PHS_VF_SCAR | PHS_VF_TEMP_BLOCK)) needs_vf = true;
+    if (hu[c] >= (float)d->lat_threshold) f |= PHS_VF_ABOVE;
+    hvf[c] = f;
+    const double cv = d->c_voxel ? d->c_voxel[o] : d->c;
+    if (cv < 0) return undo(PHS_ERR_ARG, "negative conductivity");
+    if (d->c_voxel && cv != d->c) hetero = true;
+    const double al = d->dt * cv / (2.0 * d->delta * d->delta);
+    hal[c] = (float)al;
+  }
+  // Spectral bound: 1 + 2 max(alpha_n deg_n) over conducting voxels.
+  for (int32_t b = 0; b < P.n_bricks; ++b) {
+    const uint64_t m = P.mask[b];
+    for (int lane = 0; lane < 64; ++lane) {
+      if (!((m >> lane) & 1)) continue;
+      const int32_t c = P.start[b] + __builtin_popcountll(m & ((1ULL << lane) - 1));
+      if (hvf[c] & PHS_VF_SCAR) continue;
+      int deg = 0;
+      const int32_t o = P.perm[c];
+      if (d->nb) {
+        for (int k = 0; k < 6; ++k) deg += d->nb[(int64_t)o * d->nb_stride + k] >= 0;
+      } else {
+        deg = 6;
+      }
+      amax_deg = std::max(amax_deg, (double)hal[c] * deg);
+    }
+  }
+  for (int64_t i = 0; i < d->n_steps; ++i) hsched[i] = d->schedule[i];
+
+  h.lmin = 1.0;
+  h.lmax = 1.0 + 2.0 * amax_deg;
+  std::vector<phs::ChebCoef> coef(kMaxSweeps);
+  SlotRec sr;
+  std::memset(&sr, 0, sizeof(sr));
+  if (h.lmax > 1.0) {
+    phs::chebyshev_coefficients(h.lmin, h.lmax, kMaxSweeps, coef.data());
+  } else {
+    for (auto& q : coef) q = phs::ChebCoef{0.0, 0.0, 0.0};
+  }
+  for (int k = 0; k < kMaxSweeps; ++k) {
+    sr.coef[k][0] = (float)coef[k].a;
+    sr.coef[k][1] = (float)coef[k].b;
+    sr.coef[k][2] = (float)coef[k].g;
+    sr.coef[k][3] = 0.0f;
+  }
+  sr.step = 0;
+  sr.n_steps = d->n_steps;
+  sr.voxel_off = vo;
+  sr.su = out ? (float*)out->samples_u : nullptr;
+  sr.sh = out ? (float*)out->samples_h : nullptr;
+  sr.lat = out ? out->lat : nullptr;
+  sr.n_voxel = N;
+  sr.active = 1;
+  sr.model = d->model;
+  sr.nspm = d->steps_per_sample;
+  sr.n_samples = d->n_samples;
+  sr.max_beats = out ? out->max_beats : 0;
+  sr.needs_vf = needs_vf ? 1 : 0;
+  sr.hetero = hetero ? 1 : 0;
+  if (d->model == 0) {
+    sr.p[0] = (float)(1.0 / d->params[0]);
+    sr.p[1] = (float)(1.0 / d->params[1]);
+    sr.p[2] = (float)(1.0 / d->params[2]);
+    sr.p[3] = (float)(1.0 / d->params[3]);
+    sr.p[4] = (float)d->params[4];
+  } else {
+    for (int i = 0; i < 5; ++i) sr.p[i] = (float)d->params[i];
+  }
+  sr.dt = (float)d->dt;
+  sr.alpha = (float)a0;
+  sr.alpha64 = a0;
+  sr.lat_thr = (float)d->lat_threshold;
+
+  if ((e = hipMalloc(&h.perm_dev, 4 * (size_t)N)) ||
+      (e = hipMalloc(&h.sched_dev, (size_t)std::max<int64_t>(d->n_steps, 1))))
+    return undo(PHS_ERR_OOM, std::string("slot hipMalloc: ") + hipGetErrorString(e));
+  sr.sched = h.sched_dev;
+  *hs = sr;
+  if ((e = hipMemcpyAsync(a->bricks + bo, hb, sz_br, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(a->U + vo, hu, sz_f, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(a->H + vo, hh, sz_f, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(a->ALPHA + vo, hal, sz_f, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(a->VF + vo, hvf, (size_t)N, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(h.perm_dev, hperm, sz_f, hipMemcpyHostToDevice, s)) ||
+      (d->n_steps > 0 &&
+       (e = hipMemcpyAsync(h.sched_dev, hsched, (size_t)d->n_steps, hipMemcpyHostToDevice, s))) ||
+      (out && out->lat && out->max_beats > 0 &&
+       (e = hipMemsetAsync(out->lat, 0xFF, 4 * (size_t)out->max_beats * N, s))) ||
+      (e = hipMemcpyAsync(a->slots + slot, hs, sz_sl, hipMemcpyHostToDevice, s)) ||
+      (e = hipEventCreateWithFlags(&h.upload_done, hipEventDisableTiming)) ||
+      (e = hipEventRecord(h.upload_done, s)))
+    return undo(PHS_ERR_HIP, std::string("slot upload: ") + hipGetErrorString(e));
+  h.used = true;
+  recompute_high(a);
+  *slot_out = slot;
+  return PHS_OK;
+}
+
+int phs_arena_remove(phs_arena* a, int32_t slot, void* stream) {
+  if (!a || slot < 0 || slot >= a->cfg.max_sims || !a->sh[slot].used)
+    return phs::fail(PHS_ERR_ARG, "invalid slot");
+  SlotHost& h = a->sh[slot];
+  hipStream_t s = (hipStream_t)stream;
+  // Stream-ordered: steps already queued on `stream` still see the simulation; later work
+  // (including an add that reuses its ranges) does not.
+  arena_invalidate<<<(h.n_bricks + 255) / 256, 256, 0, s>>>(a->bricks, h.brick_off, h.n_bricks);
+  arena_deactivate<<<1, 1, 0, s>>>(a->slots, slot);
+  PHS_HIP_CHECK(hipGetLastError());
+  DeferredFree g;
+  PHS_HIP_CHECK(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+  PHS_HIP_CHECK(hipEventRecord(g.done, s));
+  g.host = h.staging;
+  g.dev[0] = h.perm_dev;
+  g.dev[1] = h.sched_dev;
+  if (h.upload_done) hipEventDestroy(h.upload_done);
+  a->graveyard.push_back(g);
+  a->vox.release(h.voxel_off, h.n_voxel);
+  a->brk.release(h.brick_off, h.n_bricks);
+  h = SlotHost();
+  recompute_high(a);
+  reap(a, false);
+  return PHS_OK;
+}
+
+int phs_arena_step(phs_arena* a, int32_t n_steps, void* stream) {
+  if (!a) return phs::fail(PHS_ERR_ARG, "handle is NULL");
+  if (n_steps < 0) return phs::fail(PHS_ERR_ARG, "n_steps < 0");
+  hipStream_t s = (hipStream_t)stream;
+  if (a->brick_high == 0) return PHS_OK;
+  for (int i = 0; i < n_steps; ++i) launch_step(a, s);
+  PHS_HIP_CHECK(hipGetLastError());
+  for (auto& h : a->sh)
+    if (h.used) h.steps_done = std::min<int64_t>(h.n_steps, h.steps_done + n_steps);
+  return PHS_OK;
+}
+
+int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream) {
+  if (!a) return phs::fail(PHS_ERR_ARG, "handle is NULL");
+  if (n_steps <= 0) return phs::fail(PHS_ERR_ARG, "n_steps must be > 0");
+  if (a->brick_high == 0) return PHS_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t grid = grid_for(a);
+  if (!a->gexec || a->g_steps != n_steps || a->g_grid != grid) {
+    if (a->gexec) hipGraphExecDestroy(a->gexec);
+    if (a->graph) hipGraphDestroy(a->graph);
+    a->gexec = nullptr;
+    a->graph = nullptr;
+    if (!a->cap_stream) PHS_HIP_CHECK(hipStreamCreateWithFlags(&a->cap_stream, hipStreamNonBlocking));
+    PHS_HIP_CHECK(hipStreamBeginCapture(a->cap_stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < n_steps; ++i) launch_step(a, a->cap_stream);
+    hipGraph_t g = nullptr;
+    PHS_HIP_CHECK(hipStreamEndCapture(a->cap_stream, &g));
+    a->graph = g;
+    PHS_HIP_CHECK(hipGraphInstantiate(&a->gexec, g, nullptr, nullptr, 0));
+    a->g_steps = n_steps;
+    a->g_grid = grid;
+  }
+  PHS_HIP_CHECK(hipGraphLaunch(a->gexec, s));
+  for (auto& h : a->sh)
+    if (h.used) h.steps_done = std::min<int64_t>(h.n_steps, h.steps_done + n_steps);
+  return PHS_OK;
+}
+
+int phs_arena_slot_info(phs_arena* a, int32_t slot, int64_t* steps_done, int32_t* n_voxel,
+                        int32_t* n_bricks, int64_t* voxel_offset) {
+  if (!a || slot < 0 || slot >= a->cfg.max_sims || !a->sh[slot].used)
+    return phs::fail(PHS_ERR_ARG, "invalid slot");
+  const SlotHost& h = a->sh[slot];
+  if (steps_done) *steps_done = h.steps_done;
+  if (n_voxel) *n_voxel = h.n_voxel;
+  if (n_bricks) *n_bricks = h.n_bricks;
+  if (voxel_offset) *voxel_offset = h.voxel_off;
+  return PHS_OK;
+}
+
+int phs_arena_perm(phs_arena* a, int32_t slot, int32_t* perm_host) {
+  if (!a || !perm_host || slot < 0 || slot >= a->cfg.max_sims || !a->sh[slot].used)
+    return phs::fail(PHS_ERR_ARG, "invalid slot");
+  const SlotHost& h = a->sh[slot];
+  std::memcpy(perm_host, h.perm.data(), 4 * h.perm.size());
+  return PHS_OK;
+}
+
+int phs_arena_state(phs_arena* a, int32_t slot, float* u, float* h, void* stream) {
+  if (!a || slot < 0 || slot >= a->cfg.max_sims || !a->sh[slot].used)
+    return phs::fail(PHS_ERR_ARG, "invalid slot");
+  const SlotHost& sl = a->sh[slot];
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid((sl.n_voxel + 255) / 256, 1);
+  if (u) scatter_rows_f32<<<grid, 256, 0, s>>>(u, a->U + sl.voxel_off, sl.perm_dev, 1, sl.n_voxel);
+  if (h) scatter_rows_f32<<<grid, 256, 0, s>>>(h, a->H + sl.voxel_off, sl.perm_dev, 1, sl.n_voxel);
+  PHS_HIP_CHECK(hipGetLastError());
+  return PHS_OK;
+}
+
+int phs_arena_slot_bounds(phs_arena* a, int32_t slot, double* lmin, double* lmax) {
+  if (!a || slot < 0 || slot >= a->cfg.max_sims || !a->sh[slot].used)
+    return phs::fail(PHS_ERR_ARG, "invalid slot");
+  if (lmin) *lmin = a->sh[slot].lmin;
+  if (lmax) *lmax = a->sh[slot].lmax;
+  return PHS_OK;
+}
+
+int phs_arena_last_residual(phs_arena* a, int32_t slot, double* rel, void* stream) {
+  if (!a || !rel || slot < 0 || slot >= a->cfg.max_sims || !a->sh[slot].used)
+    return phs::fail(PHS_ERR_ARG, "invalid slot");
+  if (!a->cfg.diag) return phs::fail(PHS_ERR_STATE, "arena created without diag");
+  hipStream_t s = (hipStream_t)stream;
+  double zero[2] = {0.0, 0.0};
+  PHS_HIP_CHECK(hipMemcpyAsync(a->diag_acc, zero, 16, hipMemcpyHostToDevice, s));
+  launch_diag(a, slot, s);
+  double acc[2];
+  PHS_HIP_CHECK(hipMemcpyAsync(acc, a->diag_acc, 16, hipMemcpyDeviceToHost, s));
+  PHS_HIP_CHECK(hipStreamSynchronize(s));
+  *rel = acc[1] > 0 ? std::sqrt(acc[0] / acc[1]) : 0.0;
+  return PHS_OK;
+}
+
+int phs_arena_destroy(phs_arena* a) {
+  if (!a) return PHS_OK;
+  hipDeviceSynchronize();
+  for (auto& h : a->sh) free_slot_host(h);
+  reap(a, true);
+  if (a->gexec) hipGraphExecDestroy(a->gexec);
+  if (a->graph) hipGraphDestroy(a->graph);
+  if (a->cap_stream) hipStreamDestroy(a->cap_stream);
+  hipFree(a->bricks);
+  hipFree(a->slots);
+  hipFree(a->U);
+  hipFree(a->H);
+  hipFree(a->R0);
+  hipFree(a->DA);
+  hipFree(a->DB);
+  hipFree(a->USTAR);
+  hipFree(a->ALPHA);
+  hipFree(a->DACC);
+  hipFree(a->VF);
+  hipFree(a->diag_acc);
+  delete a;
+  return PHS_OK;
+}
+
+int phs_scatter_rows_f32(float* dst, const float* src, const int32_t* perm, int64_t rows, int64_t n,
+                         void* stream) {
+  if (!dst || !src || !perm || rows < 0 || n < 0) return phs::fail(PHS_ERR_ARG, "bad arguments");
+  if (rows == 0 || n == 0) return PHS_OK;
+  const dim3 grid((unsigned)((n + 255) / 256), (unsigned)std::min<int64_t>(rows, 65535));
+  scatter_rows_f32<<<grid, 256, 0, (hipStream_t)stream>>>(dst, src, perm, rows, n);
+  PHS_HIP_CHECK(hipGetLastError());
+  return PHS_OK;
+}
+
+int phs_scatter_rows_i32(int32_t* dst, const int32_t* src, const int32_t* perm, int64_t rows,
+                         int64_t n, void* stream) {
+  if (!dst || !src || !perm || rows < 0 || n < 0) return phs::fail(PHS_ERR_ARG, "bad arguments");
+  if (rows == 0 || n == 0) return PHS_OK;
+  const dim3 grid((unsigned)((n + 255) / 256), (unsigned)std::min<int64_t>(rows, 65535));
+  scatter_rows_i32<<<grid, 256, 0, (hipStream_t)stream>>>(dst, src, perm, rows, n);
+  PHS_HIP_CHECK(hipGetLastError());
+  return PHS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+"""Drop-in for simulation/simulation_gpu.py.
+
+``compute`` keeps the reference signature and return values
+(simulation/simulation_gpu.py:261, :409):
+
+    sim_u_voxel, sim_h_voxel, physical_time = compute(
+        n_voxel, P_2d, geometry_data, simulation_parameters, arrhythmia_parameters)
+
+with float32 (samples x N) arrays in the caller's voxel order.  Keyword-only
+extras select the device form:
+
+    precision="fp32"   bricked FP32 arena (default; reaction and diffusion in
+                       FP32, CN right-hand side in FP64)
+    precision="fp64"   plain FP64 form (reference algorithm, tight CN solve)
+    sweeps, refine, refine_sweeps, brick   solver settings of the FP32 form
+    return_lat=True    also return the per-voxel activation-time table
+                       (max_beats x N step indices, -1 = none)
+    device_out=True    return torch device tensors instead of NumPy arrays
+
+Both forms run on the GPU through libphs.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .. import runtime
+
+
+def compute(n_voxel, P_2d, geometry_data, simulation_parameters, arrhythmia_parameters, *,
+            precision: str = "fp32", sweeps: int = 16, refine: int = 0, refine_sweeps: int = 8,
+            brick=(4, 4, 4), plain_tol: float = 1e-15, return_lat: bool = False,
+            device_out: bool = False):
+    prep = runtime.prepare(n_voxel, P_2d, geometry_data, simulation_parameters,
+                           arrhythmia_parameters, need_lattice=(precision != "fp64"))
+    ptime = prep.physical_time()
+    if precision == "fp64":
+        sim = runtime.PlainSim(prep, tol=plain_tol, sample_fp64=False)
+        sim.run()
+        su, sh, lat = sim.su, sim.sh, sim.lat
+    elif precision == "fp32":
+        solver = runtime.SolverConfig(sweeps=sweeps, refine=refine, refine_sweeps=refine_sweeps,
+                                      brick=tuple(brick))
+        arena = runtime.Arena(max_voxels=n_voxel, max_sims=1, solver=solver,
+                              max_bricks=n_voxel)
+        slot = arena.add(prep, samples=True, sample_h=True)
+        arena.step(prep.n_steps, graph=False)
+        su, sh, lat = arena.outputs(slot)
+    else:
+        raise ValueError(f"Unknown precision: {precision}. Use 'fp32' or 'fp64'")
+    T = len(ptime)
+    su, sh = su[:T], sh[:T]
+    if device_out:
+        res = (su, sh, ptime)
+    else:
+        res = (su.cpu().numpy(), sh.cpu().numpy(), ptime)
+    if return_lat:
+        res = res + (lat if device_out else lat.cpu().numpy(),)
+    return res

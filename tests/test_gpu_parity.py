@@ -1,0 +1,104 @@
+"""GPU parity of the simulation hot path against the FP64 oracle (through the C ABI).
+
+Bars (written in each test):
+  * plain FP64 form: max |dVm| / max |Vm_oracle| <= 1e-9, LAT identical;
+  * bricked FP32 form: LAT within 1 dt of the oracle for every voxel and beat;
+  * alone vs packed with other simulations: bitwise identical.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import run_oracle, small_sheet, small_shell
+
+pytestmark = pytest.mark.gpu
+
+
+def _prep(w):
+    from pyheartsim_amd import runtime
+    P = w["P_2d"]
+    return runtime.prepare(P.shape[0], P, w["geometry"], w["sim"], w["arr"])
+
+
+def _plain(w, probes=None):
+    from pyheartsim_amd import runtime
+    prep = _prep(w)
+    sim = runtime.PlainSim(prep, tol=1e-15, probes=probes)
+    sim.run()
+    return prep, sim
+
+
+def _lat_ok(lat_a, lat_b, tol_steps):
+    a = np.asarray(lat_a, dtype=np.int64)
+    b = np.asarray(lat_b, dtype=np.int64)
+    assert np.array_equal(a >= 0, b >= 0), "different voxels activated"
+    m = a >= 0
+    return int(np.max(np.abs(a[m] - b[m]))) if m.any() else 0
+
+
+@pytest.mark.parametrize("scar", [0, 2])
+def test_plain_fp64_matches_oracle_shell(cuda, scar):
+    w = small_shell(n_scar=scar, t_final=40.0)
+    ref = run_oracle(w, max_beats=7)
+    prep, sim = _plain(w)
+    print(f"shell N={prep.n_voxel} steps={prep.n_steps} sweeps={sim.info()}", flush=True)
+    su = sim.su.cpu().numpy()
+    T = ref["u"].shape[0]
+    assert su.shape[0] == T
+    err = np.max(np.abs(su - ref["u"])) / np.max(np.abs(ref["u"]))
+    print("rel Vm err", err)
+    assert err <= 1e-9
+    sh = sim.sh.cpu().numpy()
+    assert np.max(np.abs(sh - ref["h"])) <= 1e-9
+    assert np.array_equal(sim.lat.cpu().numpy().astype(np.int64), ref["lat"][:7])
+
+
+def test_plain_fp64_matches_oracle_sheet_with_temp_block(cuda):
+    w = small_sheet(n=32, n_steps=400)
+    nf = np.zeros(w["P_2d"].shape[0], dtype=np.int64)
+    ijk = w["geometry"]["ijk"]
+    nf[(ijk[:, 0] == 20) & (ijk[:, 1] < 20)] = 3
+    w["arr"]["node_flag"] = nf
+    probes = w["probes"]
+    ref = run_oracle(w, probes=probes)
+    prep, sim = _plain(w, probes=probes)
+    err = np.max(np.abs(sim.su.cpu().numpy() - ref["u"])) / np.max(np.abs(ref["u"]))
+    assert err <= 1e-9
+    perr = np.max(np.abs(sim.probe_u.cpu().numpy() - ref["probe_u"]))
+    assert perr <= 1e-9 * np.max(np.abs(ref["probe_u"]))
+    assert np.array_equal(sim.lat.cpu().numpy().astype(np.int64), ref["lat"][:7])
+
+
+def test_fp32_arena_lat_within_one_step(cuda):
+    from pyheartsim_amd import runtime
+    w = small_shell(t_final=40.0)
+    ref = run_oracle(w, max_beats=7)
+    prep = _prep(w)
+    arena = runtime.Arena(prep.n_voxel, 1, runtime.SolverConfig(sweeps=16))
+    slot = arena.add(prep)
+    arena.step(prep.n_steps)
+    su, _, lat = arena.outputs(slot)
+    d = _lat_ok(lat.cpu().numpy(), ref["lat"][:7], 1)
+    vm = np.max(np.abs(su.cpu().numpy() - ref["u"]))
+    print(f"fp32 arena: max LAT diff {d} steps, max |dVm| {vm:.3e}")
+    assert d <= 1
+
+
+def test_arena_alone_equals_mixed_and_graph(cuda):
+    from pyheartsim_amd import runtime
+    wa = small_shell(t_final=20.0, seed=3)
+    wb = small_shell(t_final=30.0, seed=5, semi_axes=(8.0, 7.0, 6.0), n_scar=1)
+    pa, pb = _prep(wa), _prep(wb)
+    solver = runtime.SolverConfig(sweeps=12)
+    alone = runtime.Arena(pa.n_voxel, 1, solver)
+    s = alone.add(pa)
+    alone.step(pa.n_steps)
+    ua, _, la = alone.outputs(s)
+    mixed = runtime.Arena(pa.n_voxel + pb.n_voxel, 2, solver)
+    sb = mixed.add(pb)
+    mixed.step(100)
+    sa = mixed.add(pa)  # admitted while the other simulation is mid-run
+    for _ in range(pa.n_steps // 50 + 1):
+        mixed.step(50, graph=True)
+    um, _, lm = mixed.outputs(sa)
+    assert np.array_equal(ua.cpu().numpy(), um.cpu().numpy())
+    assert np.array_equal(la.cpu().numpy(), lm.cpu().numpy())
