@@ -1,0 +1,265 @@
+#!/usr/bin/env python
+"""Throughput of the simulation hot path on one MI355X.
+
+Workload: the Mitchell-Schaeffer left-atrium-like shell (ellipsoid 25/20/18 mm,
+3 mm wall, 4 vein holes of 6 mm, 0.4 mm voxels), dt = 0.05 ms, focal pacing at
+CL 500 ms for 2000 ms (40 000 steps), 64 basket electrodes at 1 kHz.
+
+Legs (all on cuda:0, one process):
+  plain    plain FP64 form, one simulation (reference algorithm, original voxel
+           order, CN solve iterated to 1e-15), the denominator of the speed-up
+  single   chosen form (bricked FP32, hipGraph steps), one simulation
+  batched  chosen form, `--batch` simulations of the shell with different seeded
+           pacing sites packed into one launch sequence -> `value`
+  validate full 40 000-step runs of plain and single: LAT error (steps), max |dVm|
+  egm      BF16 hi/lo MFMA electrogram of the single run's samples vs the FP32
+           MFMA electrogram of the plain run's samples
+  cpu      the FP64 oracle with the reference's BiCGSTAB stop rule on a bounded
+           window of steps from a mid-activation state (host cores)
+
+A step is one dt of reaction + full CN solve for every voxel of every resident
+simulation; value = voxel-steps/s over exactly --steps timed steps of the
+batched leg, inputs resident in HBM.  One JSON line goes to stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(seed_sites):
+    from pyheartsim_amd import configs, geometry as geo, runtime
+    g = geo.shell((25.0, 20.0, 18.0), 3.0, 0.4, 4, 6.0, 0)
+    preps = []
+    work = None
+    for s in seed_sites:
+        w = configs.shell_ms(seed=s, geometry=g)
+        P = configs.build_P_2d(g, w["sim"], w["hm"])
+        preps.append(runtime.prepare(P.shape[0], P, g, w["sim"], w["arr"]))
+        if work is None:
+            work = w
+            work["P_2d"] = P
+    return g, work, preps
+
+
+def timed(torch, fn, steps, warmup, chunk):
+    done = 0
+    while done < warmup:
+        n = min(chunk, warmup - done)
+        fn(n)
+        done += n
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        fn(n)
+        done += n
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def lat_error(a, b):
+    a = a.astype(np.int64)
+    b = b.astype(np.int64)
+    both = (a >= 0) & (b >= 0)
+    mismatch = int(np.count_nonzero((a >= 0) != (b >= 0)))
+    d = np.abs(a[both] - b[both])
+    return dict(max=int(d.max()) if d.size else 0,
+                p99=float(np.percentile(d, 99)) if d.size else 0.0,
+                activation_mismatch=mismatch, compared=int(d.size))
+
+
+def cpu_baseline(work, u0, h0, step0, budget_s):
+    from oracle import sim_oracle
+    P = work["P_2d"]
+    n = P.shape[0]
+    kw = dict(solver="bicgstab_ref", record_samples=False, u0=u0, h0=h0, step0=step0)
+    t0 = time.perf_counter()
+    sim_oracle.simulate(n, P, work["geometry"], work["sim"], work["arr"], n_steps=1, **kw)
+    t1 = time.perf_counter()
+    sim_oracle.simulate(n, P, work["geometry"], work["sim"], work["arr"], n_steps=3, **kw)
+    per = max((time.perf_counter() - t1) - (t1 - t0), 1e-3) / 2
+    steps = max(2, min(2000, int(budget_s / per)))
+    t0 = time.perf_counter()
+    sim_oracle.simulate(n, P, work["geometry"], work["sim"], work["arr"], n_steps=steps, **kw)
+    dt = time.perf_counter() - t0
+    return dict(value=n * steps / dt, unit="voxel-steps/s", cores=1, kind="port",
+                sample=f"{steps} steps (from step {step0}, mid-activation) of the {n}-voxel shell: "
+                       "NumPy/SciPy FP64 restatement of legacy/compute_simulation_cpu.py with "
+                       "BiCGSTAB(atol 1e-6, rtol 1e-5, maxiter 1000), single-threaded")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--sweeps", type=int, default=16)
+    ap.add_argument("--graph-steps", type=int, default=50)
+    ap.add_argument("--no-validate", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the simulations "
+              "shard by request across GPUs, see the multi-GPU design notes.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pyheartsim_amd import runtime
+    from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    t_setup = time.perf_counter()
+    g, work, preps = build_workload(range(args.batch))
+    prep = preps[0]
+    N = prep.n_voxel
+    S_full = prep.n_steps
+    log(f"workload: N={N} steps={S_full} batch={args.batch} setup {time.perf_counter()-t_setup:.1f}s")
+    K, W = args.steps, args.warmup
+    G = args.graph_steps
+    solver = runtime.SolverConfig(sweeps=args.sweeps, brick=(4, 4, 4))
+    res = {}
+
+    # ---- plain FP64, one simulation
+    plain = runtime.PlainSim(prep, tol=1e-15, samples=not args.no_validate, sample_h=False,
+                             sample_fp64=False)
+    sweeps_plain = plain.info()[0]
+    t = timed(torch, plain.run, K, W, 200)
+    res["plain"] = dict(voxel_steps_per_s=N * K / t, ms_per_step=1e3 * t / K, sweeps=sweeps_plain)
+    log(f"plain fp64: {res['plain']}")
+    u_mid, h_mid = plain.state()
+    u_mid = u_mid.cpu().numpy()
+    h_mid = h_mid.cpu().numpy()
+    step_mid = W + K
+
+    # ---- chosen form, one simulation
+    single = runtime.Arena(N, 1, solver)
+    slot = single.add(prep, samples=not args.no_validate)
+    t = timed(torch, lambda n: single.step(n, graph=True), (K // G) * G, (W // G) * G, G)
+    ks = (K // G) * G
+    res["single"] = dict(voxel_steps_per_s=N * ks / t, ms_per_step=1e3 * t / ks)
+    log(f"chosen single: {res['single']}")
+
+    # ---- validation: run both to the end of the 2000 ms simulation
+    if not args.no_validate:
+        t0 = time.perf_counter()
+        plain.run()
+        while single.steps_done(slot) < S_full:
+            single.step(G, graph=True)
+        torch.cuda.synchronize()
+        log(f"validation runs finished in {time.perf_counter()-t0:.1f}s")
+        su_c, _, lat_c = single.outputs(slot)
+        lat_p = plain.lat.cpu().numpy()
+        lat_cn = lat_c.cpu().numpy()
+        le = lat_error(lat_cn, lat_p)
+        T = prep.n_saved()
+        vm = float((su_c[:T] - plain.su[:T]).abs().max().item())
+        res["validate"] = dict(lat_err_steps=le, max_abs_dVm=vm, beats=int((lat_p[:, 0] >= 0).sum()))
+        log(f"validation: {res['validate']}")
+        # electrogram: chosen (BF16 hi/lo on compact-order samples) vs FP32 MFMA on plain samples
+        el = work["electrodes"]
+        c = work["hm"]["c_voxel"]
+        lf_c = LeadField(el, g["voxel"], None, c, g["Delta"], g["neighbor_id_2d"],
+                         perm=single.slots[slot].perm)
+        lf_p = LeadField(el, g["voxel"], None, c, g["Delta"], g["neighbor_id_2d"])
+        samples_c = single.slots[slot].su[:T]
+        e_ref = lf_p.run(plain.su[:T], precision="fp32")
+        e_c = lf_c.run(samples_c, precision="bf16x3")
+        torch.cuda.synchronize()
+        rel = float(((e_c - e_ref).norm() / e_ref.norm()).item())
+        st = torch.cuda.current_stream()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(st)
+        reps = 5
+        for _ in range(reps):
+            lf_c.run(samples_c, precision="bf16x3", normalize=False)
+        ev1.record(st)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        E = el.shape[0]
+        useful = 2.0 * E * T * N
+        res["egm"] = dict(rel_l2=rel, ms=ms, electrodes=E, samples=T,
+                          useful_tflops=useful / ms / 1e9,
+                          mfma_issued_tflops=3 * useful / ms / 1e9,
+                          reference_equivalent_tflops=2.0 * E * T * 3 * N / ms / 1e9)
+        log(f"egm: {res['egm']}")
+        del plain, single
+        torch.cuda.empty_cache()
+
+    # ---- chosen form, batched
+    B = len(preps)
+    arena = runtime.Arena(N * B, B, solver)
+    for p in preps:
+        arena.add(p, samples=False)
+    cnt = arena.counts()
+    t = timed(torch, lambda n: arena.step(n, graph=True), ks, (W // G) * G, G)
+    value = cnt["voxels"] * ks / t
+    ms_step = 1e3 * t / ks
+    prof = arena.profile(50)
+    vf = cnt["vf_voxels"]
+    bytes_sweep = cnt["voxels"] * 16 + vf + cnt["bricks"] * 48
+    achieved = bytes_sweep / (prof["sweep"] * 1e-3) / 1e9
+    log(f"batched: {value:.4g} voxel-steps/s, {ms_step:.4f} ms/step, kernels {prof}, counts {cnt}")
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(work, u_mid, h_mid, step_mid, args.cpu_seconds)
+        log(f"cpu: {cpu}")
+
+    line = {
+        "metric": "voxel-steps/s at LAT err<=1dt (chosen vs plain FP64); p50/p99 sim latency",
+        "value": value,
+        "unit": "voxel-steps/s",
+        "n_gpus": 1,
+        "steps": ks,
+        "warmup": (W // G) * G,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"shell_ms: MS ellipsoidal LA shell 25/20/18 mm, 3 mm wall, 4 vein holes, "
+                        f"0.4 mm voxels (N={N}), dt 0.05 ms, CL 500 ms, x{B} simulations packed "
+                        f"(different seeded pacing sites)",
+            "n_voxel": N, "batch": B, "steps_per_sim": S_full, "sweeps": args.sweeps,
+            "brick": [4, 4, 4], "graph_steps": G,
+        },
+        "speedup_vs_plain_fp64": value / res["plain"]["voxel_steps_per_s"],
+        "plain_fp64": res["plain"],
+        "single": res["single"],
+        "validate": res.get("validate"),
+        "egm": res.get("egm"),
+        "latency_ms": {"p50": None, "p99": None},
+        "roofline": {
+            "bound": "hbm", "kernel": "arena_sweep (Chebyshev sweep, FP32)",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "bytes_per_launch": bytes_sweep, "avg_launch_ms": prof["sweep"],
+            "kernels_ms": prof,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

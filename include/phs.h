@@ -100,6 +100,15 @@ int phs_version(void);
 const char* phs_last_error(void);
 int phs_device_count(int32_t* count);
 
+/* Host only (no device work): the Morton brick plan of a voxel set for brick shape
+ * brick[3] (bx*by*bz = 64).  perm [n_voxel] receives the compact -> caller order,
+ * *n_bricks the brick count, masks (NULL or capacity n_voxel) the per-brick occupancy.
+ * When nb is not NULL the face-neighbour table is verified against the lattice
+ * (PHS_ERR_UNSUPPORTED on mismatch).  Replaces nothing in the reference: it is the
+ * voxel reordering that the device forms add. */
+int phs_plan_build(int32_t n_voxel, const int32_t* ijk, const int32_t* nb, int32_t nb_stride,
+                   const int32_t* brick, int32_t* perm, int32_t* n_bricks, uint64_t* masks);
+
 /* ------------------------------------------------------------------------------------------
  * Plain FP64 form: the reference algorithm (simulation/simulation_gpu.py:358-407 with the
  * FP64 arithmetic of legacy/compute_simulation_cpu.py:144-228) on one GPU, voxels in the
@@ -166,6 +175,14 @@ int phs_arena_slot_bounds(phs_arena* a, int32_t slot, double* lambda_min, double
 /* FP64 relative residual ||b - A x|| / ||b|| of the last CN solve of a slot, recomputed on
  * the device from the stored correction (diagnostic; refine > 0 keeps the correction). */
 int phs_arena_last_residual(phs_arena* a, int32_t slot, double* rel_residual, void* stream);
+/* Average duration (ms, HIP events on `stream`) of the reaction, right-hand-side and
+ * Chebyshev-sweep kernels over `reps` launches each: ms[0..2].  Simulation state is not
+ * changed (the reaction writes to scratch, the sweep runs in its intermediate mode). */
+int phs_arena_profile(phs_arena* a, int32_t reps, double* ms, void* stream);
+/* Resident voxels, bricks, voxels of simulations that read the flag byte in the sweeps
+ * (scar / temporary block), and bricks covered by one launch. */
+int phs_arena_counts(phs_arena* a, int64_t* voxels, int64_t* bricks, int64_t* vf_voxels,
+                     int64_t* launch_bricks);
 int phs_arena_destroy(phs_arena* a);
 
 /* Row permutation: dst[r][perm[i]] = src[r][i] for r < n_rows, i < n (float). */

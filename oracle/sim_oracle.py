@@ -159,7 +159,7 @@ def pacing_constants(arr, sim):
 # ---------------------------------------------------------------- time loop
 def simulate(n_voxel, P_2d, geometry_data, simulation_parameters, arrhythmia_parameters, *,
              solver="direct", n_steps=None, lat_threshold=0.13, max_beats=8, probes=None,
-             record_samples=True, sample_dtype=np.float64):
+             record_samples=True, sample_dtype=np.float64, u0=None, h0=None, step0=0):
     """simulation/simulation_gpu.py:261-409 in FP64.
 
     Returns a dict with ``u``, ``h`` (samples x N), ``physical_time``,
@@ -183,8 +183,11 @@ def simulate(n_voxel, P_2d, geometry_data, simulation_parameters, arrhythmia_par
     s1 = arr["s1_pacing_voxel_id"]
     s2 = arr.get("s2_pacing_voxel_id", [])
 
-    u = np.zeros(n_voxel)
-    h = np.ones(n_voxel) if model == 0 else np.zeros(n_voxel)
+    u = np.zeros(n_voxel) if u0 is None else np.array(u0, dtype=np.float64)
+    if h0 is not None:
+        h = np.array(h0, dtype=np.float64)
+    else:
+        h = np.ones(n_voxel) if model == 0 else np.zeros(n_voxel)
     n_samples = int(np.round(t_final)) if model == 0 else int(np.round(t_final * ts))
 
     P_perm = P.copy()
@@ -212,7 +215,7 @@ def simulate(n_voxel, P_2d, geometry_data, simulation_parameters, arrhythmia_par
     probe_u = np.zeros((total, len(probes)))
     J = np.zeros(n_voxel)
     id_save = 0
-    for step in range(total):
+    for step in range(step0, step0 + total):
         model_time = step * dt
         active = cn_tb if (model_time >= 0 and model_time < 350 * dt) else cn
         J.fill(0.0)
@@ -232,7 +235,7 @@ def simulate(n_voxel, P_2d, geometry_data, simulation_parameters, arrhythmia_par
             beats[cross] += 1
         above = now
         if len(probes):
-            probe_u[step] = u[probes]
+            probe_u[step - step0] = u[probes]
         save = (step % nspm == 0) if nspm > 0 else True
         if save and id_save < n_samples:
             if record_samples:

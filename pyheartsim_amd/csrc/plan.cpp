@@ -150,3 +150,21 @@ int build_plan(int32_t n, const int32_t* ijk, const int32_t* nb, int32_t nb_stri
 }
 
 }  // namespace phs
+
+#include "phs_internal.h"
+
+extern "C" int phs_plan_build(int32_t n, const int32_t* ijk, const int32_t* nb, int32_t nb_stride,
+                              const int32_t* brick, int32_t* perm, int32_t* n_bricks,
+                              uint64_t* masks) {
+  if (!brick || !perm || !n_bricks) return phs::fail(PHS_ERR_ARG, "NULL argument");
+  if (nb && nb_stride < 6) return phs::fail(PHS_ERR_ARG, "nb_stride must be >= 6");
+  phs::BrickPlan P;
+  std::string err;
+  const int rc = phs::build_plan(n, ijk, nb, nb_stride, brick[0], brick[1], brick[2], P, err);
+  if (rc == -2) return phs::fail(PHS_ERR_UNSUPPORTED, err);
+  if (rc) return phs::fail(PHS_ERR_ARG, err);
+  std::copy(P.perm.begin(), P.perm.end(), perm);
+  *n_bricks = P.n_bricks;
+  if (masks) std::copy(P.mask.begin(), P.mask.end(), masks);
+  return PHS_OK;
+}

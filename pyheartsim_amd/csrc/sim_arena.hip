@@ -74,6 +74,8 @@ struct ArenaDev {
   SlotRec* slots;
   float* U;
   float* H;
+  float* UO;  // reaction outputs (= U, H except when timing the kernel in isolation)
+  float* HO;
   float* R0;
   double* DACC;
   float* USTAR;
@@ -190,8 +192,8 @@ __global__ void __launch_bounds__(kBlock) arena_reaction(ArenaDev A) {
   rhs32(S, u + 0.5f * dt * k2u, h + 0.5f * dt * k2h, J, k3u, k3h);
   rhs32(S, u + dt * k3u, h + dt * k3h, J, k4u, k4h);
   const float s6 = dt * (1.0f / 6.0f);
-  A.U[idx] = u + s6 * (k1u + 2.0f * k2u + 2.0f * k3u + k4u);
-  A.H[idx] = h + s6 * (k1h + 2.0f * k2h + 2.0f * k3h + k4h);
+  A.UO[idx] = u + s6 * (k1u + 2.0f * k2u + 2.0f * k3u + k4u);
+  A.HO[idx] = h + s6 * (k1h + 2.0f * k2h + 2.0f * k3h + k4h);
 }
 
 // r0 = 2 alpha sum (u*_m - u*_n), in FP64 from FP32 u*.
@@ -429,6 +431,7 @@ struct SlotHost {
   int32_t* perm_dev = nullptr;
   uint8_t* sched_dev = nullptr;
   double lmin = 1.0, lmax = 1.0;
+  bool needs_vf = false;
   void* staging = nullptr;  // pinned upload buffer, freed after its copies complete
   hipEvent_t upload_done = nullptr;
 };
@@ -469,6 +472,8 @@ ArenaDev dev_view(const phs_arena* a) {
   A.slots = a->slots;
   A.U = a->U;
   A.H = a->H;
+  A.UO = a->U;
+  A.HO = a->H;
   A.R0 = a->R0;
   A.DACC = a->DACC;
   A.USTAR = a->USTAR;
@@ -559,6 +564,46 @@ void launch_step(phs_arena* a, hipStream_t s) {
   else if (b[0] == 4 && b[1] == 8) launch_step_shape<4, 8, 2>(a, s);
   else if (b[0] == 16) launch_step_shape<16, 4, 1>(a, s);
   else launch_step_shape<4, 4, 4>(a, s);
+}
+
+// Average duration (ms) of each kernel class over `reps` launches, timed with HIP events
+// on `s`.  The reaction writes into scratch (DA, DB) and the sweep runs in MID mode, so no
+// simulation state changes.
+template <int BX, int BY, int BZ>
+hipError_t profile_shape(phs_arena* a, int reps, double* ms, hipStream_t s) {
+  ArenaDev A = dev_view(a);
+  const int grid = grid_for(a);
+  hipEvent_t ev[4];
+  for (auto& e : ev)
+    if (hipError_t r = hipEventCreate(&e)) return r;
+  ArenaDev R = A;
+  R.UO = a->DA;
+  R.HO = a->DB;
+  hipEventRecord(ev[0], s);
+  for (int i = 0; i < reps; ++i) arena_reaction<<<grid, kBlock, 0, s>>>(R);
+  hipEventRecord(ev[1], s);
+  for (int i = 0; i < reps; ++i) arena_rhs<BX, BY, BZ><<<grid, kBlock, 0, s>>>(A);
+  hipEventRecord(ev[2], s);
+  for (int i = 0; i < reps; ++i)
+    arena_sweep<BX, BY, BZ, MODE_MID><<<grid, kBlock, 0, s>>>(A, 3, a->DA, 0, a->DB, 2, a->DB);
+  hipEventRecord(ev[3], s);
+  hipError_t r = hipEventSynchronize(ev[3]);
+  for (int i = 0; i < 3; ++i) {
+    float t = 0.0f;
+    hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+    ms[i] = (double)t / reps;
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  return r;
+}
+
+hipError_t profile_kernels(phs_arena* a, int reps, double* ms, hipStream_t s) {
+  const int* b = a->cfg.brick;
+  if (b[0] == 8 && b[1] == 8) return profile_shape<8, 8, 1>(a, reps, ms, s);
+  if (b[0] == 8 && b[1] == 4) return profile_shape<8, 4, 2>(a, reps, ms, s);
+  if (b[0] == 4 && b[1] == 8) return profile_shape<4, 8, 2>(a, reps, ms, s);
+  if (b[0] == 16) return profile_shape<16, 4, 1>(a, reps, ms, s);
+  return profile_shape<4, 4, 4>(a, reps, ms, s);
 }
 
 void launch_diag(phs_arena* a, int slot, hipStream_t s) {
@@ -775,6 +820,7 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
 
   h.lmin = 1.0;
   h.lmax = 1.0 + 2.0 * amax_deg;
+  h.needs_vf = needs_vf;
   std::vector<phs::ChebCoef> coef(kMaxSweeps);
   SlotRec sr;
   std::memset(&sr, 0, sizeof(sr));
@@ -959,6 +1005,32 @@ int phs_arena_last_residual(phs_arena* a, int32_t slot, double* rel, void* strea
   PHS_HIP_CHECK(hipMemcpyAsync(acc, a->diag_acc, 16, hipMemcpyDeviceToHost, s));
   PHS_HIP_CHECK(hipStreamSynchronize(s));
   *rel = acc[1] > 0 ? std::sqrt(acc[0] / acc[1]) : 0.0;
+  return PHS_OK;
+}
+
+int phs_arena_profile(phs_arena* a, int32_t reps, double* ms, void* stream) {
+  if (!a || !ms || reps <= 0) return phs::fail(PHS_ERR_ARG, "invalid profile arguments");
+  if (a->brick_high == 0) return phs::fail(PHS_ERR_STATE, "arena is empty");
+  PHS_HIP_CHECK(profile_kernels(a, reps, ms, (hipStream_t)stream));
+  PHS_HIP_CHECK(hipGetLastError());
+  return PHS_OK;
+}
+
+int phs_arena_counts(phs_arena* a, int64_t* voxels, int64_t* bricks, int64_t* vf_voxels,
+                     int64_t* launch_bricks) {
+  if (!a) return phs::fail(PHS_ERR_ARG, "handle is NULL");
+  int64_t v = 0, b = 0;
+  for (auto& h : a->sh)
+    if (h.used) {
+      v += h.n_voxel;
+      b += h.n_bricks;
+    }
+  if (voxels) *voxels = v;
+  if (bricks) *bricks = b;
+  if (vf_voxels) *vf_voxels = 0;
+  for (auto& h : a->sh)
+    if (h.used && h.needs_vf && vf_voxels) *vf_voxels += h.n_voxel;
+  if (launch_bricks) *launch_bricks = (int64_t)grid_for(a) * kWaves;
   return PHS_OK;
 }
 

@@ -146,6 +146,21 @@ def prepare(n_voxel, P_2d, geometry_data, simulation_parameters, arrhythmia_para
         schedule=np.ascontiguousarray(schedule, np.uint8))
 
 
+def brick_plan(ijk, nb=None, brick=(4, 4, 4)):
+    """Host-only Morton brick plan (phs_plan_build): (perm, masks[n_bricks])."""
+    lib = _lib.load()
+    ijk = np.ascontiguousarray(ijk, dtype=np.int32)
+    n = ijk.shape[0]
+    nb6 = None if nb is None else np.ascontiguousarray(np.asarray(nb)[:, :6], dtype=np.int32)
+    b = np.ascontiguousarray(brick, dtype=np.int32)
+    perm = np.empty(n, dtype=np.int32)
+    masks = np.empty(n, dtype=np.uint64)
+    nbk = C.c_int32()
+    _lib.check(lib.phs_plan_build(n, _lib.ptr(ijk), _lib.ptr(nb6), 6, _lib.ptr(b), _lib.ptr(perm),
+                                  C.byref(nbk), _lib.ptr(masks)), "phs_plan_build")
+    return perm, masks[:nbk.value]
+
+
 # --------------------------------------------------------------------- plain FP64 form
 class PlainSim:
     """Plain FP64 form on one GPU (phs_plain_*).  Outputs in caller voxel order."""
@@ -308,6 +323,19 @@ class Arena:
         _lib.check(self.lib.phs_arena_last_residual(self.handle, slot, C.byref(r), _lib.stream_ptr()),
                    "phs_arena_last_residual")
         return r.value
+
+    def profile(self, reps: int = 50, stream=None):
+        """Average kernel durations (ms): reaction, rhs, sweep (HIP events)."""
+        ms = (C.c_double * 3)()
+        _lib.check(self.lib.phs_arena_profile(self.handle, reps, ms, _lib.stream_ptr(stream)),
+                   "phs_arena_profile")
+        return dict(reaction=ms[0], rhs=ms[1], sweep=ms[2])
+
+    def counts(self):
+        v, b, f, lb = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        _lib.check(self.lib.phs_arena_counts(self.handle, C.byref(v), C.byref(b), C.byref(f),
+                                             C.byref(lb)))
+        return dict(voxels=v.value, bricks=b.value, vf_voxels=f.value, launch_bricks=lb.value)
 
     def state(self, slot: int):
         torch = self.torch

@@ -1,0 +1,76 @@
+"""Pins of the pacing rule (simulation/pacing.py:136-170) with hand-derived step tables,
+and agreement of the product's per-step schedule with the oracle's rule."""
+import numpy as np
+import pytest
+
+from oracle import sim_oracle as so
+from pyheartsim_amd import configs, geometry as geo
+from pyheartsim_amd.simulation import pacing
+
+
+def _runs(bits):
+    on = np.nonzero(bits)[0]
+    if not len(on):
+        return []
+    br = np.nonzero(np.diff(on) > 1)[0]
+    starts = np.concatenate([[on[0]], on[br + 1]])
+    ends = np.concatenate([on[br], [on[-1]]])
+    return [(int(s), int(e - s + 1)) for s, e in zip(starts, ends)]
+
+
+def test_focal_schedule_hand_derived():
+    # start 10 ms, CL 500 ms, 5 ms pulse, dt 0.05 ms, 2000 ms.  Beat 1 covers t in [10, 15]
+    # (steps 200-300, 101 steps).  From beat 2 on, the repeated subtraction
+    # `while t - start > CL` leaves t = 510.0 unreduced (510 - 10 > 500 is false), so
+    # 510.00 ms is outside the window and the beat starts one step later: 100 steps each.
+    g = geo.sheet(4, 4)
+    sim, arr, _ = configs.assign_simulation_parameters(
+        g, 0, pacing_cycle_length=500.0, t_final=2000.0, dt=0.05)
+    sched = pacing.pacing_schedule(arr, sim)
+    assert len(sched) == 40000
+    assert _runs(sched & pacing.SCHED_S1) == [(200, 101), (10201, 100), (20201, 100), (30201, 100)]
+    assert not np.any(sched & pacing.SCHED_S2)
+    tb = np.nonzero(sched & pacing.SCHED_TEMP_BLOCK)[0]
+    assert tb[0] == 0 and tb[-1] == 349 and len(tb) == 350
+
+
+@pytest.mark.parametrize("flag", [0, 1, 2, 3, 4, 5, 6])
+def test_schedule_matches_oracle_rule(flag):
+    g = geo.sheet(4, 4)
+    for model in (0, 1):
+        sim, arr, _ = configs.assign_simulation_parameters(
+            g, 0, [1], heart_model_flag=model, arrhythmia_flag=flag, t_final=700.0, dt=0.5,
+            pacing_cycle_length=170.0, pacing_start_time=10.0, s1_t=20.0, s1_s2_delta_t=250.0)
+        sched = pacing.pacing_schedule(arr, sim)
+        s1_t, dur, s2_t = so.pacing_constants(arr, sim)
+        for step in range(len(sched)):
+            on1, on2 = so.stim_on(arr, sim, flag, step * sim["dt"], s1_t, dur, s2_t)
+            assert bool(sched[step] & 1) == on1 and bool(sched[step] & 2) == on2, (flag, model, step)
+
+
+def test_single_s1_and_s2_windows():
+    g = geo.sheet(4, 4)
+    sim, arr, _ = configs.assign_simulation_parameters(
+        g, 0, [1], arrhythmia_flag=1, t_final=400.0, dt=0.5, s1_t=0.0, s1_s2_delta_t=250.0)
+    sched = pacing.pacing_schedule(arr, sim)
+    assert _runs(sched & 1) == [(0, 11)]       # t in [0, 5] at dt 0.5
+    assert _runs(sched & 2) == [(500, 11)]     # t in [250, 255]
+    sim5, arr5, _ = configs.assign_simulation_parameters(
+        g, 0, [1], arrhythmia_flag=5, heart_model_flag=1, t_final=600.0, dt=0.5)
+    s5 = pacing.pacing_schedule(arr5, sim5)
+    # AP: dt = 0.5/6, duration 5/6, f2 = 300/6 = 50 model units
+    assert _runs(s5 & 1)[0][0] == 0
+    assert _runs(s5 & 2)[0][0] == int(np.ceil(50 / (0.5 / 6) - 1e-9))
+
+
+def test_apply_pacing_dropin_fills_sites():
+    g = geo.sheet(4, 4)
+    sim, arr, _ = configs.assign_simulation_parameters(g, np.array([3, 4]), [7], arrhythmia_flag=1,
+                                                       dt=0.5, s1_s2_delta_t=250.0)
+    J, s1, s2, s1_t, mag, dur, s2_t = pacing.assign_pacing_parameters(arr, 1, 16, None, sim)
+    assert mag == 1 and dur == 5 and s2_t == 250.0
+    J = pacing.apply_pacing(arr, sim, 1, 251.0, J, s1, s2, s1_t, mag, dur, s2_t, None, None, None)
+    assert list(np.nonzero(J)[0]) == [7]
+    J.fill(0)
+    J = pacing.apply_pacing(arr, sim, 1, 2.0, J, s1, s2, s1_t, mag, dur, s2_t, None, None, None)
+    assert list(np.nonzero(J)[0]) == [3, 4]
