@@ -86,23 +86,27 @@ def lat_error(a, b):
 
 
 def cpu_baseline(work, u0, h0, step0, budget_s):
-    from oracle import sim_oracle
-    P = work["P_2d"]
-    n = P.shape[0]
-    kw = dict(solver="bicgstab_ref", record_samples=False, u0=u0, h0=h0, step0=step0)
+    """C/OpenMP FP64 restatement of legacy/compute_simulation_cpu.py (reference BiCGSTAB
+    stop rule) on the host cores, from a mid-activation state, for ~budget_s seconds."""
+    from oracle import cpu_oracle
+    threads = min(16, cpu_oracle.threads_available())
+    cs = cpu_oracle.CpuSim(work["P_2d"], work["geometry"], work["sim"], work["arr"])
+    cs.u[:] = u0
+    cs.h[:] = h0
+    cs.step = step0
     t0 = time.perf_counter()
-    sim_oracle.simulate(n, P, work["geometry"], work["sim"], work["arr"], n_steps=1, **kw)
-    t1 = time.perf_counter()
-    sim_oracle.simulate(n, P, work["geometry"], work["sim"], work["arr"], n_steps=3, **kw)
-    per = max((time.perf_counter() - t1) - (t1 - t0), 1e-3) / 2
-    steps = max(2, min(2000, int(budget_s / per)))
+    cs.run(5, threads)
+    per = max(time.perf_counter() - t0, 1e-4) / 5
+    steps = max(5, min(20000, int(budget_s / per)))
     t0 = time.perf_counter()
-    sim_oracle.simulate(n, P, work["geometry"], work["sim"], work["arr"], n_steps=steps, **kw)
+    fails = cs.run(steps, threads)
     dt = time.perf_counter() - t0
-    return dict(value=n * steps / dt, unit="voxel-steps/s", cores=1, kind="port",
-                sample=f"{steps} steps (from step {step0}, mid-activation) of the {n}-voxel shell: "
-                       "NumPy/SciPy FP64 restatement of legacy/compute_simulation_cpu.py with "
-                       "BiCGSTAB(atol 1e-6, rtol 1e-5, maxiter 1000), single-threaded")
+    n = cs.n
+    return dict(value=n * steps / dt, unit="voxel-steps/s", cores=threads, kind="port",
+                sample=f"{steps} steps from step {step0} (mid-activation) of the {n}-voxel shell: "
+                       "C/OpenMP FP64 restatement of legacy/compute_simulation_cpu.py, CSR L, "
+                       "BiCGSTAB(atol 1e-6, rtol 1e-5, maxiter 1000) from x0 = u*",
+                maxiter_hits=int(fails), seconds=dt)
 
 
 def main():
@@ -111,7 +115,10 @@ def main():
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--sweeps", type=int, default=16)
+    ap.add_argument("--sweeps", type=int, default=6)
+    ap.add_argument("--guess", type=int, default=1)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="run a device copy of this size first (HBM counter calibration)")
     ap.add_argument("--graph-steps", type=int, default=50)
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -134,7 +141,16 @@ def main():
     log(f"workload: N={N} steps={S_full} batch={args.batch} setup {time.perf_counter()-t_setup:.1f}s")
     K, W = args.steps, args.warmup
     G = args.graph_steps
-    solver = runtime.SolverConfig(sweeps=args.sweeps, brick=(4, 4, 4))
+    solver = runtime.SolverConfig(sweeps=args.sweeps, guess=args.guess, brick=(4, 4, 4))
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.rand(n, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        del src, dst
     res = {}
 
     # ---- plain FP64, one simulation
@@ -214,7 +230,10 @@ def main():
     ms_step = 1e3 * t / ks
     prof = arena.profile(50)
     vf = cnt["vf_voxels"]
-    bytes_sweep = cnt["voxels"] * 16 + vf + cnt["bricks"] * 48
+    # algorithmic bytes of one Chebyshev sweep launch: read d_k, d_{k-1}, r (FP32) and write
+    # d_{k+1} per voxel, the flag byte of scar/temporary-block simulations, and the 96-byte
+    # brick records (neighbour values gathered across brick faces are re-reads, not counted)
+    bytes_sweep = cnt["voxels"] * 16 + vf + cnt["bricks"] * 96
     achieved = bytes_sweep / (prof["sweep"] * 1e-3) / 1e9
     log(f"batched: {value:.4g} voxel-steps/s, {ms_step:.4f} ms/step, kernels {prof}, counts {cnt}")
 
@@ -241,7 +260,7 @@ def main():
                         f"0.4 mm voxels (N={N}), dt 0.05 ms, CL 500 ms, x{B} simulations packed "
                         f"(different seeded pacing sites)",
             "n_voxel": N, "batch": B, "steps_per_sim": S_full, "sweeps": args.sweeps,
-            "brick": [4, 4, 4], "graph_steps": G,
+            "guess": args.guess, "brick": [4, 4, 4], "graph_steps": G,
         },
         "speedup_vs_plain_fp64": value / res["plain"]["voxel_steps_per_s"],
         "plain_fp64": res["plain"],

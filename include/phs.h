@@ -148,6 +148,9 @@ typedef struct phs_arena_config {
   int32_t xcd_remap;       /* 1 = map consecutive bricks to one XCD */
   int32_t diag;            /* 1 = keep u* and the correction of the last step so that
                               phs_arena_last_residual can recompute the CN residual */
+  int32_t bricks_per_wave; /* bricks handled by one wavefront: 1 or 2 */
+  int32_t guess;           /* warm start of the CN correction: 0 none, 1 previous step's
+                              correction, 2 linear extrapolation of the last two */
 } phs_arena_config;
 
 int phs_arena_create(const phs_arena_config* cfg, phs_arena** handle);

@@ -1,0 +1,190 @@
+/*
+ * FP64 CPU restatement of the reference's CPU simulation path, OpenMP over host cores.
+ * TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may call it, and only as the checker / the reported CPU baseline.
+ *
+ * Follows legacy/compute_simulation_cpu.py:
+ *   reaction   :144-209  RK4, same operation order (numba prange passes -> omp for)
+ *   operator   :21-111   CSR L with L[n,n] = c_n/(4 Delta^2) * -(sum_k P[n,k]),
+ *                        L[n,m_k] = c_n/(4 Delta^2) * P[n,k] for present face neighbours
+ *                        (P = 4 delta_k for D = I; the anisotropy terms are zero)
+ *   CN step    :113-142  A = I - dt/2 L, b = u* + dt/2 L u*, BiCGSTAB from x0 = u*
+ *                        stopping at ||r|| < max(rtol ||b||, atol) (scipy's rule; the
+ *                        reference passes atol = 1e-6 and keeps rtol = 1e-5), maxiter 1000
+ * with the pacing schedule, scar / temporary block and sampling of
+ * simulation/simulation_gpu.py:261-409 supplied as per-step bytes by the caller.
+ */
+#include <math.h>
+#include <omp.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+  int32_t n;
+  int32_t model;
+  double p[5];
+  const double* c;        /* [n] conductivity, scar already zeroed */
+  const uint8_t* tb;      /* [n] 1 = temporary block voxel, or NULL */
+  const int32_t* nb;      /* [n][6] face neighbours, -1 = none */
+  const uint8_t* s1;      /* [n] S1 site */
+  const uint8_t* s2;      /* [n] S2 site */
+  double dt, delta;
+  double rtol, atol;
+  int32_t maxiter;
+} cpu_desc;
+
+typedef struct {
+  int32_t n;
+  int32_t* rowptr;
+  int32_t* col;
+  double* val;
+} csr;
+
+static void rhs(int model, const double* p, double u, double h, double J, double* du, double* dh) {
+  if (model == 0) {
+    *du = (h * (u * u) * (1 - u)) / p[2] - u / p[3] + J;
+    *dh = (u < p[4]) ? (1 - h) / p[0] : -h / p[1];
+  } else {
+    *du = -p[0] * u * (u - p[1]) * (u - 1) - u * h + J;
+    *dh = (p[2] + p[3] * h / (u + p[4])) * (-h - p[0] * u * (u - p[1] - 1));
+  }
+}
+
+static void build_L(const cpu_desc* d, int temp_block_on, csr* L) {
+  const int n = d->n;
+  L->n = n;
+  L->rowptr = (int32_t*)malloc(sizeof(int32_t) * (n + 1));
+  L->col = (int32_t*)malloc(sizeof(int32_t) * 7 * (size_t)n);
+  L->val = (double*)malloc(sizeof(double) * 7 * (size_t)n);
+  int32_t k = 0;
+  for (int i = 0; i < n; ++i) {
+    L->rowptr[i] = k;
+    double c = d->c[i];
+    if (temp_block_on && d->tb && d->tb[i]) c = 0.0;
+    const double Dc = c / (4 * d->delta * d->delta);
+    double psum = 0.0;
+    for (int q = 0; q < 6; ++q) psum += d->nb[6 * (size_t)i + q] >= 0 ? 4.0 : 0.0;
+    L->col[k] = i;
+    L->val[k++] = Dc * -psum;
+    for (int q = 0; q < 6; ++q) {
+      const int m = d->nb[6 * (size_t)i + q];
+      if (m >= 0) {
+        L->col[k] = m;
+        L->val[k++] = Dc * 4.0;
+      }
+    }
+  }
+  L->rowptr[n] = k;
+}
+
+static void free_L(csr* L) {
+  free(L->rowptr);
+  free(L->col);
+  free(L->val);
+}
+
+/* y = x - s * (L x)  (s = dt/2 gives A x; s = -dt/2 gives b from u*) */
+static void apply(const csr* L, double s, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < L->n; ++i) {
+    double acc = 0.0;
+    for (int k = L->rowptr[i]; k < L->rowptr[i + 1]; ++k) acc += L->val[k] * x[L->col[k]];
+    y[i] = x[i] - s * acc;
+  }
+}
+
+static double dot(const double* a, const double* b, int n) {
+  double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (int i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+/* Unpreconditioned BiCGSTAB (the algorithm of scipy.sparse.linalg.bicgstab with M = I). */
+static int bicgstab(const csr* L, double hdt, const double* b, double* x, double rtol, double atol,
+                    int maxiter, double* work) {
+  const int n = L->n;
+  double *r = work, *rt = work + n, *p = work + 2 * (size_t)n, *v = work + 3 * (size_t)n,
+         *s = work + 4 * (size_t)n, *t = work + 5 * (size_t)n;
+  const double bn = sqrt(dot(b, b, n));
+  const double tol = fmax(atol, rtol * bn);
+  apply(L, hdt, x, v);
+#pragma omp parallel for
+  for (int i = 0; i < n; ++i) {
+    r[i] = b[i] - v[i];
+    rt[i] = r[i];
+  }
+  if (sqrt(dot(r, r, n)) < tol) return 0;
+  double rho_prev = 1.0, alpha = 1.0, omega = 1.0;
+  for (int it = 0; it < maxiter; ++it) {
+    const double rho = dot(rt, r, n);
+    if (rho == 0.0) return -1;
+    if (it == 0) {
+      memcpy(p, r, sizeof(double) * n);
+    } else {
+      const double beta = (rho / rho_prev) * (alpha / omega);
+#pragma omp parallel for
+      for (int i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+    }
+    apply(L, hdt, p, v);
+    alpha = rho / dot(rt, v, n);
+#pragma omp parallel for
+    for (int i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
+    if (sqrt(dot(s, s, n)) < tol) {
+#pragma omp parallel for
+      for (int i = 0; i < n; ++i) x[i] += alpha * p[i];
+      return 0;
+    }
+    apply(L, hdt, s, t);
+    omega = dot(t, s, n) / dot(t, t, n);
+#pragma omp parallel for
+    for (int i = 0; i < n; ++i) {
+      x[i] += alpha * p[i] + omega * s[i];
+      r[i] = s[i] - omega * t[i];
+    }
+    if (sqrt(dot(r, r, n)) < tol) return 0;
+    rho_prev = rho;
+  }
+  return 1;
+}
+
+/* Advance (u, h) by n_steps steps; sched[i] bits: 1 S1 on, 2 S2 on, 4 temporary block.
+ * Returns the number of CN solves that hit maxiter. */
+int cpu_simulate(const cpu_desc* d, const uint8_t* sched, int64_t n_steps, double* u, double* h,
+                 int threads) {
+  if (threads > 0) omp_set_num_threads(threads);
+  const int n = d->n;
+  csr L0, L1;
+  build_L(d, 0, &L0);
+  build_L(d, 1, &L1);
+  double* us = (double*)malloc(sizeof(double) * n);
+  double* b = (double*)malloc(sizeof(double) * n);
+  double* work = (double*)malloc(sizeof(double) * 6 * (size_t)n);
+  int fails = 0;
+  const double dt = d->dt;
+  for (int64_t st = 0; st < n_steps; ++st) {
+    const uint8_t sc = sched[st];
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+      const double J = (((sc & 1) && d->s1[i]) || ((sc & 2) && d->s2[i])) ? 1.0 : 0.0;
+      double k1u, k1h, k2u, k2h, k3u, k3h, k4u, k4h;
+      rhs(d->model, d->p, u[i], h[i], J, &k1u, &k1h);
+      rhs(d->model, d->p, u[i] + 0.5 * dt * k1u, h[i] + 0.5 * dt * k1h, J, &k2u, &k2h);
+      rhs(d->model, d->p, u[i] + 0.5 * dt * k2u, h[i] + 0.5 * dt * k2h, J, &k3u, &k3h);
+      rhs(d->model, d->p, u[i] + dt * k3u, h[i] + dt * k3h, J, &k4u, &k4h);
+      us[i] = u[i] + (dt / 6.0) * (k1u + 2 * k2u + 2 * k3u + k4u);
+      h[i] = h[i] + (dt / 6.0) * (k1h + 2 * k2h + 2 * k3h + k4h);
+    }
+    const csr* L = (sc & 4) ? &L1 : &L0;
+    apply(L, -dt / 2.0, us, b);
+    memcpy(u, us, sizeof(double) * n);
+    if (bicgstab(L, dt / 2.0, b, u, d->rtol, d->atol, d->maxiter, work)) ++fails;
+  }
+  free(us);
+  free(b);
+  free(work);
+  free_L(&L0);
+  free_L(&L1);
+  return fails;
+}

@@ -64,6 +64,8 @@ class ArenaConfig(C.Structure):
         ("refine_sweeps", C.c_int32),
         ("xcd_remap", C.c_int32),
         ("diag", C.c_int32),
+        ("bricks_per_wave", C.c_int32),
+        ("guess", C.c_int32),
     ]
 
 

@@ -1,0 +1,505 @@
+// Device code of the bricked FP32 arena (included by sim_arena.hip only).
+//
+// A brick is 64 lattice cells handled by one wavefront (lane l <-> cell
+// (l % BX, (l / BX) % BY, l / (BX*BY))).  The brick record (96 B, one scalar load) holds the
+// brick's occupancy mask and compact start and those of its six face-neighbour bricks, so
+// every lane's loads depend on that one scalar load only, and all of them are issued before
+// any value is used.  Neighbour presence is computed on the wave-uniform 64-bit masks with
+// scalar shifts and used as lane masks (inverse ballot); compact indices come from mbcnt.
+// In-brick neighbour values move between lanes with ds_bpermute (__shfl); cross-brick
+// values are gathered (lanes without a cross neighbour re-load their own value, which hits
+// the same cache line and keeps the code branch-free).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "phs_internal.h"
+
+namespace arena {
+
+constexpr int kWaves = 4;
+constexpr int kBlock = 64 * kWaves;
+constexpr int kMaxSweeps = 64;
+
+struct BrickRec {
+  uint64_t mask;       // occupied cells
+  int32_t start;       // compact index of the first occupied cell
+  int32_t slot;        // simulation slot, -1 = unused brick
+  uint64_t nmask[6];   // face-neighbour bricks' masks (0 = no brick)
+  int32_t nstart[6];   // face-neighbour bricks' starts
+  int32_t pad[2];
+};
+static_assert(sizeof(BrickRec) == 96, "brick record layout");
+
+struct SlotRec {
+  int64_t step;       // steps done
+  int64_t n_steps;
+  int64_t voxel_off;
+  int64_t cur_row;    // sample row written by this step, -1 = none
+  const uint8_t* sched;
+  float* su;
+  float* sh;
+  int32_t* lat;
+  int32_t n_voxel;
+  int32_t live;       // resident and step < n_steps
+  int32_t cur_sched;  // schedule byte of this step
+  int32_t model;
+  int32_t nspm;
+  int32_t n_samples;
+  int32_t max_beats;
+  int32_t needs_vf;   // scar or temporary-block voxels present
+  int32_t hetero;     // per-voxel alpha in the ALPHA array
+  float p[5];         // MS: 1/tau_open, 1/tau_close, 1/tau_in, 1/tau_out, v_gate; AP: k,a,eps0,mu1,mu2
+  float dt;
+  float alpha;
+  float lat_thr;
+  double alpha64;
+  float coef[kMaxSweeps][4];  // a, b, g, -; coef[0][2] = 1/theta
+};
+
+struct ArenaDev {
+  const BrickRec* __restrict__ bricks;
+  SlotRec* slots;
+  float* U;
+  float* H;
+  float* UO;  // reaction outputs (= U, H except when the kernel is timed in isolation)
+  float* HO;
+  float* R0;
+  double* DACC;
+  float* USTAR;
+  uint8_t* VF;
+  const float* ALPHA;
+  float* D1;  // CN correction of the previous step (warm start)
+  float* D2;  // correction of the step before (linear extrapolation)
+  int32_t n_bricks;
+  int32_t xcd;
+  int32_t diag;
+};
+
+enum { MODE_MID = 0, MODE_FINAL_X = 1, MODE_DACC_SET = 2, MODE_DACC_ADD = 3, MODE_DACC_ADD_X = 4 };
+
+// Warm start of the CN correction d (x = u* + d): 0, the previous step's correction, or
+// the linear extrapolation 2 d_{n-1} - d_{n-2}.
+__device__ __forceinline__ float guess_of(int guess, float d1, float d2) {
+  return guess == 0 ? 0.0f : (guess == 1 ? d1 : 2.0f * d1 - d2);
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
+// number of bits of m below this lane
+__device__ __forceinline__ int mbcnt64(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// First brick of this wavefront.  With `xcd`, the round-robin dispatch of workgroups over
+// the 8 XCDs is undone so that each XCD works on a contiguous run of bricks and the halo
+// gathers hit that XCD's L2 (the grid is then a multiple of 8).
+template <int NB>
+__device__ __forceinline__ int first_brick(int xcd) {
+  int blk = blockIdx.x;
+  if (xcd) {
+    const int per = gridDim.x >> 3;
+    blk = (blk & 7) * per + (blk >> 3);
+  }
+  return __builtin_amdgcn_readfirstlane((blk * kWaves + (int)(threadIdx.x >> 6)) * NB);
+}
+
+// Lane patterns of a brick shape: INB[d] = lanes whose neighbour in direction d
+// (0 +x, 1 -x, 2 +y, 3 -y, 4 +z, 5 -z) lies in the same brick.
+template <int BX, int BY, int BZ>
+struct Shape {
+  static_assert(BX * BY * BZ == 64, "64 cells per brick");
+  static constexpr int B[3] = {BX, BY, BZ};
+  static constexpr int St[3] = {1, BX, BX * BY};
+  static constexpr uint64_t inb(int d) {
+    uint64_t m = 0;
+    for (int l = 0; l < 64; ++l) {
+      const int c[3] = {l % BX, (l / BX) % BY, l / (BX * BY)};
+      const int ax = d >> 1;
+      const bool in = (d & 1) ? c[ax] > 0 : c[ax] < B[ax] - 1;
+      if (in) m |= 1ULL << l;
+    }
+    return m;
+  }
+  static constexpr int step(int d) { return St[d >> 1]; }
+  static constexpr int wrap(int d) { return (B[d >> 1] - 1) * St[d >> 1]; }
+};
+
+// Per-brick, per-lane stencil addressing (everything derived from the brick record).
+template <int BX, int BY, int BZ>
+struct Stencil {
+  using SH = Shape<BX, BY, BZ>;
+  uint64_t pin[6];  // lanes whose d-neighbour is an occupied cell of this brick
+  uint64_t pcr[6];  // lanes whose d-neighbour is an occupied cell of the neighbour brick
+  int gaddr[6];     // gather address of the cross neighbour (own index where none)
+  int idx;          // compact index (the brick's first voxel on empty lanes)
+  bool valid;
+
+  __device__ __forceinline__ void init(const BrickRec& br) {
+    valid = lane_bit(br.mask);
+    idx = br.start + (valid ? mbcnt64(br.mask) : 0);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      const int st = SH::step(d), k = SH::wrap(d);
+      const uint64_t IN = SH::inb(d);
+      pin[d] = ((d & 1) ? (br.mask << st) : (br.mask >> st)) & IN & br.mask;
+      const uint64_t nm = br.nmask[d];
+      // the cell of the neighbour brick adjacent to lane l is l - k (+ face) or l + k (- face)
+      const uint64_t sh = (d & 1) ? (nm >> k) : (nm << k);
+      pcr[d] = sh & ~IN & br.mask;
+      const int below = (d & 1) ? __popcll(nm & ((1ULL << k) - 1ULL)) : 0;
+      gaddr[d] = lane_bit(pcr[d]) ? br.nstart[d] + mbcnt64(sh) + below : idx;
+    }
+  }
+
+  template <typename V>
+  __device__ __forceinline__ void gather(const V* __restrict__ src, V (&g)[6]) const {
+#pragma unroll
+    for (int d = 0; d < 6; ++d) g[d] = src[gaddr[d]];
+  }
+
+  // sum over present face neighbours of (v_m - v_n) in type T; own value v, cross values g
+  // scaled by sc.
+  template <typename T, typename V>
+  __device__ __forceinline__ T sum(V v, const V (&g)[6], V sc, int lane) const {
+    T s = (T)0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      const int st = SH::step(d);
+      const V pv = __shfl(v, ((d & 1) ? lane - st : lane + st) & 63);
+      const V m = lane_bit(pin[d]) ? pv : (lane_bit(pcr[d]) ? g[d] * sc : v);
+      s += (T)m - (T)v;
+    }
+    return s;
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ T alpha_of(const ArenaDev& A, const SlotRec& S, int idx) {
+  T a = sizeof(T) == 8 ? (T)S.alpha64 : (T)S.alpha;
+  if (S.hetero) a = (T)A.ALPHA[idx];
+  if (S.needs_vf) {
+    const uint8_t f = A.VF[idx];
+    if ((f & PHS_VF_SCAR) || ((S.cur_sched & PHS_SCHED_TEMP_BLOCK) && (f & PHS_VF_TEMP_BLOCK)))
+      a = (T)0;
+  }
+  return a;
+}
+
+__device__ __forceinline__ void rhs32(const SlotRec& S, float u, float h, float J, float& du,
+                                      float& dh) {
+  if (S.model == 0) {
+    du = h * u * u * (1.0f - u) * S.p[2] - u * S.p[3] + J;
+    dh = (u < S.p[4]) ? (1.0f - h) * S.p[0] : -h * S.p[1];
+  } else {
+    const float k = S.p[0], a = S.p[1];
+    du = -k * u * (u - a) * (u - 1.0f) - u * h + J;
+    dh = (S.p[2] + S.p[3] * h * __builtin_amdgcn_rcpf(u + S.p[4])) * (-h - k * u * (u - a - 1.0f));
+  }
+}
+
+// RK4 (simulation/simulation_gpu.py:71-130) in FP32, NB bricks per wavefront.  Loads are
+// issued on the brick record alone (a finished simulation's bricks still address resident
+// data); the slot's liveness is checked before anything is stored.
+template <int NB>
+__global__ void __launch_bounds__(kBlock) k_reaction(ArenaDev A) {
+  const int b0 = first_brick<NB>(A.xcd);
+  float u[NB], h[NB];
+  uint8_t f[NB];
+  int idx[NB];
+  bool on[NB], valid[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int b = b0 + j;
+    on[j] = b < A.n_bricks && A.bricks[b].slot >= 0;
+    if (on[j]) {
+      const BrickRec& br = A.bricks[b];
+      valid[j] = lane_bit(br.mask);
+      idx[j] = br.start + (valid[j] ? mbcnt64(br.mask) : 0);
+      u[j] = A.U[idx[j]];
+      h[j] = A.H[idx[j]];
+      f[j] = A.VF[idx[j]];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    if (!on[j]) continue;
+    const SlotRec& S = A.slots[A.bricks[b0 + j].slot];
+    if (!S.live || !valid[j]) continue;
+    const int sc = S.cur_sched;
+    const float J = (((sc & PHS_SCHED_S1) && (f[j] & PHS_VF_S1)) ||
+                     ((sc & PHS_SCHED_S2) && (f[j] & PHS_VF_S2)))
+                        ? 1.0f
+                        : 0.0f;
+    const float dt = S.dt;
+    float k1u, k1h, k2u, k2h, k3u, k3h, k4u, k4h;
+    rhs32(S, u[j], h[j], J, k1u, k1h);
+    rhs32(S, u[j] + 0.5f * dt * k1u, h[j] + 0.5f * dt * k1h, J, k2u, k2h);
+    rhs32(S, u[j] + 0.5f * dt * k2u, h[j] + 0.5f * dt * k2h, J, k3u, k3h);
+    rhs32(S, u[j] + dt * k3u, h[j] + dt * k3h, J, k4u, k4h);
+    const float s6 = dt * (1.0f / 6.0f);
+    A.UO[idx[j]] = u[j] + s6 * (k1u + 2.0f * k2u + 2.0f * k3u + k4u);
+    A.HO[idx[j]] = h[j] + s6 * (k1h + 2.0f * k2h + 2.0f * k3h + k4h);
+  }
+}
+
+// Right-hand side of the correction solve A e = r, x = u* + g + e, with the warm start g:
+//   r = dt L u* - A g = 2 alpha sum (u*_m - u*_n) - (g_n - alpha sum (g_m - g_n)),
+// formed in FP64 from FP32 u* and g.
+template <int BX, int BY, int BZ, int NB, int GUESS>
+__global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
+  const int b0 = first_brick<NB>(A.xcd);
+  const int lane = lane_id();
+  Stencil<BX, BY, BZ> st[NB];
+  bool on[NB];
+  float un[NB], d1[NB], d2[NB];
+  float gu[NB][6], g1[NB][6], g2[NB][6];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int b = b0 + j;
+    on[j] = b < A.n_bricks && A.bricks[b].slot >= 0;
+    if (on[j]) {
+      st[j].init(A.bricks[b]);
+      un[j] = A.U[st[j].idx];
+      st[j].gather(A.U, gu[j]);
+      if (GUESS >= 1) {
+        d1[j] = A.D1[st[j].idx];
+        st[j].gather((const float*)A.D1, g1[j]);
+      }
+      if (GUESS == 2) {
+        d2[j] = A.D2[st[j].idx];
+        st[j].gather((const float*)A.D2, g2[j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    if (!on[j]) continue;
+    const float uv = st[j].valid ? un[j] : 0.0f;
+    const double s = st[j].template sum<double>(uv, gu[j], 1.0f, lane);
+    double sg = 0.0, gn = 0.0;
+    if (GUESS >= 1) {
+      float q[6];
+#pragma unroll
+      for (int d = 0; d < 6; ++d) q[d] = guess_of(GUESS, g1[j][d], GUESS == 2 ? g2[j][d] : 0.0f);
+      const float gv = st[j].valid ? guess_of(GUESS, d1[j], GUESS == 2 ? d2[j] : 0.0f) : 0.0f;
+      sg = st[j].template sum<double>(gv, q, 1.0f, lane);
+      gn = gv;
+    }
+    const SlotRec& S = A.slots[A.bricks[b0 + j].slot];
+    if (!st[j].valid || !S.live) continue;
+    const double a = alpha_of<double>(A, S, st[j].idx);
+    A.R0[st[j].idx] = (float)(2.0 * a * s - (gn - a * sg));
+  }
+}
+
+// Activation time, samples and state of the last pass of a step.
+__device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotRec& S, int idx, float x) {
+  A.U[idx] = x;
+  const uint8_t f = A.VF[idx];
+  const bool now = x >= S.lat_thr;
+  const bool above = (f & PHS_VF_ABOVE) != 0;
+  const int64_t local = (int64_t)idx - S.voxel_off;
+  if (now && !above) {
+    int bt = (f & PHS_VF_BEAT_MASK) >> PHS_VF_BEAT_SHIFT;
+    if (bt < S.max_beats && S.lat) S.lat[(int64_t)bt * S.n_voxel + local] = (int32_t)S.step;
+    bt = bt < 7 ? bt + 1 : 7;
+    A.VF[idx] = (uint8_t)((f & 0x0F) | (bt << PHS_VF_BEAT_SHIFT) | PHS_VF_ABOVE);
+  } else if (!now && above) {
+    A.VF[idx] = (uint8_t)(f & ~PHS_VF_ABOVE);
+  }
+  if (S.cur_row >= 0) {
+    const int64_t o = S.cur_row * S.n_voxel + local;
+    if (S.su) S.su[o] = x;
+    if (S.sh) S.sh[o] = A.H[idx];
+  }
+}
+
+// One Chebyshev sweep d_{k+1} = a_k d_k + b_k d_{k-1} + g_k (r - A d_k), A d = d - alpha sum.
+// cur_kind 1: d_k = r / theta (k = 1).  prev_kind 0: d_{k-1} = 0; 1: r / theta; 2: prev[].
+// GUESS: warm start of the correction (see guess_of); the final modes add it back and keep
+// the step's total correction in D1 (and the previous one in D2) for the next step.
+template <int BX, int BY, int BZ, int MODE, int NB, int GUESS>
+__global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, const float* cur, int cur_kind,
+                                                  const float* prev, int prev_kind, float* out) {
+  const int b0 = first_brick<NB>(A.xcd);
+  const int lane = lane_id();
+  const float* src = cur_kind ? A.R0 : cur;
+  Stencil<BX, BY, BZ> st[NB];
+  bool on[NB];
+  float dn[NB], r0[NB], dp[NB];
+  float g6[NB][6];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int b = b0 + j;
+    on[j] = b < A.n_bricks && A.bricks[b].slot >= 0;
+    if (on[j]) {
+      st[j].init(A.bricks[b]);
+      const int i = st[j].idx;
+      dn[j] = src[i];
+      r0[j] = A.R0[i];
+      dp[j] = prev_kind == 2 ? prev[i] : 0.0f;
+      st[j].gather(src, g6[j]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    if (!on[j]) continue;
+    const SlotRec& S = A.slots[A.bricks[b0 + j].slot];
+    const float inv_theta = S.coef[0][2];
+    const float sc = cur_kind ? inv_theta : 1.0f;
+    const float d = st[j].valid ? dn[j] * sc : 0.0f;
+    const float s = st[j].template sum<float>(d, g6[j], sc, lane);
+    if (!st[j].valid || !S.live) continue;
+    const int i = st[j].idx;
+    const float a = alpha_of<float>(A, S, i);
+    const float Ad = d - a * s;
+    const float dpv = prev_kind == 1 ? r0[j] * inv_theta : dp[j];
+    const float dnew = S.coef[k][0] * d + S.coef[k][1] * dpv + S.coef[k][2] * (r0[j] - Ad);
+    if (MODE == MODE_MID) {
+      out[i] = dnew;
+    } else if (MODE == MODE_FINAL_X) {
+      const float us = A.U[i];
+      float dt = dnew;
+      if (GUESS >= 1) {
+        const float o1 = A.D1[i];
+        dt += guess_of(GUESS, o1, GUESS == 2 ? A.D2[i] : 0.0f);
+        if (GUESS == 2) A.D2[i] = o1;
+        A.D1[i] = dt;
+      }
+      if (A.diag) {
+        A.USTAR[i] = us;
+        A.DACC[i] = (double)dt;
+      }
+      finish_voxel(A, S, i, us + dt);
+    } else if (MODE == MODE_DACC_SET) {
+      double base = 0.0;
+      if (GUESS >= 1) base = guess_of(GUESS, A.D1[i], GUESS == 2 ? A.D2[i] : 0.0f);
+      A.DACC[i] = base + (double)dnew;
+    } else if (MODE == MODE_DACC_ADD) {
+      A.DACC[i] += (double)dnew;
+    } else {
+      const double acc = A.DACC[i] + (double)dnew;
+      const float us = A.U[i];
+      A.DACC[i] = acc;
+      if (GUESS >= 1) {
+        if (GUESS == 2) A.D2[i] = A.D1[i];
+        A.D1[i] = (float)acc;
+      }
+      if (A.diag) A.USTAR[i] = us;
+      finish_voxel(A, S, i, (float)((double)us + acc));
+    }
+  }
+}
+
+// FP64 residual of the accumulated correction: r = 2 alpha sum(u*_m - u*_n) - A dacc.
+template <int BX, int BY, int BZ>
+__global__ void __launch_bounds__(kBlock) k_residual(ArenaDev A) {
+  const int b = first_brick<1>(A.xcd);
+  if (b >= A.n_bricks || A.bricks[b].slot < 0) return;
+  const int lane = lane_id();
+  Stencil<BX, BY, BZ> st;
+  st.init(A.bricks[b]);
+  const float un = A.U[st.idx];
+  const double dn = A.DACC[st.idx];
+  float gu[6];
+  double gd[6];
+  st.gather((const float*)A.U, gu);
+  st.gather((const double*)A.DACC, gd);
+  const double su = st.template sum<double>(st.valid ? un : 0.0f, gu, 1.0f, lane);
+  const double sd = st.template sum<double>(st.valid ? dn : 0.0, gd, 1.0, lane);
+  const SlotRec& S = A.slots[A.bricks[b].slot];
+  if (!st.valid || !S.live) return;
+  const double a = alpha_of<double>(A, S, st.idx);
+  A.R0[st.idx] = (float)(2.0 * a * su - (dn - a * sd));
+}
+
+// Advance the per-simulation counters and stage the next step's schedule byte and sample row.
+__global__ void k_advance(SlotRec* slots, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n || !slots[s].live) return;
+  SlotRec& S = slots[s];
+  const int64_t t = S.step + 1;
+  S.step = t;
+  if (t >= S.n_steps) {
+    S.live = 0;
+    return;
+  }
+  S.cur_sched = S.sched[t];
+  int64_t row = -1;
+  if (S.nspm > 0) {
+    if (t % S.nspm == 0) row = t / S.nspm;
+  } else {
+    row = t;
+  }
+  S.cur_row = row < S.n_samples ? row : -1;
+}
+
+__global__ void k_deactivate(SlotRec* slots, int slot) { slots[slot].live = 0; }
+
+__global__ void k_invalidate(BrickRec* bricks, int64_t b0, int64_t nb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nb) bricks[b0 + i].slot = -1;
+}
+
+// Diagnostic: sum of r^2 and b^2 with b = u* + alpha sum(u*_m - u*_n), x = u* + dacc,
+// r = b - (x - alpha sum(x_m - x_n)), for the last completed step of one slot.
+template <int BX, int BY, int BZ>
+__global__ void __launch_bounds__(kBlock) k_diag(ArenaDev A, int slot_sel, double* acc) {
+  const int b = first_brick<1>(A.xcd);
+  if (b >= A.n_bricks || A.bricks[b].slot != slot_sel) return;
+  const SlotRec& S = A.slots[slot_sel];
+  const int lane = lane_id();
+  Stencil<BX, BY, BZ> st;
+  st.init(A.bricks[b]);
+  const float us = st.valid ? A.USTAR[st.idx] : 0.0f;
+  const double dn = st.valid ? A.DACC[st.idx] : 0.0;
+  float gu[6];
+  double gd[6];
+  st.gather((const float*)A.USTAR, gu);
+  st.gather((const double*)A.DACC, gd);
+  const double su = st.template sum<double>(us, gu, 1.0f, lane);
+  const double sd = st.template sum<double>(dn, gd, 1.0, lane);
+  double r2 = 0.0, b2 = 0.0;
+  if (st.valid) {
+    const uint8_t sc = S.step > 0 ? S.sched[S.step - 1] : 0;
+    double a = S.hetero ? (double)A.ALPHA[st.idx] : S.alpha64;
+    if (S.needs_vf) {
+      const uint8_t f = A.VF[st.idx];
+      if ((f & PHS_VF_SCAR) || ((sc & PHS_SCHED_TEMP_BLOCK) && (f & PHS_VF_TEMP_BLOCK))) a = 0.0;
+    }
+    const double bb = (double)us + a * su;
+    const double r = bb - ((double)us + dn - a * (su + sd));
+    r2 = r * r;
+    b2 = bb * bb;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    r2 += __shfl_xor(r2, o);
+    b2 += __shfl_xor(b2, o);
+  }
+  if (lane == 0) {
+    atomicAdd(acc, r2);
+    atomicAdd(acc + 1, b2);
+  }
+}
+
+__global__ void k_scatter_f32(float* __restrict__ dst, const float* __restrict__ src,
+                              const int32_t* __restrict__ perm, int64_t rows, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t r = blockIdx.y; r < rows; r += gridDim.y)
+    if (i < n) dst[r * n + perm[i]] = src[r * n + i];
+}
+
+__global__ void k_scatter_i32(int32_t* __restrict__ dst, const int32_t* __restrict__ src,
+                              const int32_t* __restrict__ perm, int64_t rows, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t r = blockIdx.y; r < rows; r += gridDim.y)
+    if (i < n) dst[r * n + perm[i]] = src[r * n + i];
+}
+
+}  // namespace arena

@@ -30,358 +30,11 @@
 #include "phs_internal.h"
 #include "plan.h"
 
+#include "arena_kernels.h"
+
+using namespace arena;
+
 namespace {
-
-constexpr int kWaves = 4;
-constexpr int kBlock = 64 * kWaves;
-constexpr int kMaxSweeps = 64;
-
-struct BrickRec {
-  uint64_t mask;
-  int32_t start;
-  int32_t slot;
-  int32_t nbb[6];
-  int32_t pad[2];
-};
-static_assert(sizeof(BrickRec) == 48, "brick record layout");
-
-struct SlotRec {
-  int64_t step;
-  int64_t n_steps;
-  int64_t voxel_off;
-  const uint8_t* sched;
-  float* su;
-  float* sh;
-  int32_t* lat;
-  int32_t n_voxel;
-  int32_t active;
-  int32_t model;
-  int32_t nspm;
-  int32_t n_samples;
-  int32_t max_beats;
-  int32_t needs_vf;  // scar or temporary-block voxels present
-  int32_t hetero;    // per-voxel alpha in the ALPHA array
-  float p[5];        // MS: 1/tau_open, 1/tau_close, 1/tau_in, 1/tau_out, v_gate; AP: k,a,eps0,mu1,mu2
-  float dt;
-  float alpha;
-  float lat_thr;
-  double alpha64;
-  float coef[kMaxSweeps][4];  // a, b, g, unused; coef[0][2] = 1/theta
-};
-
-struct ArenaDev {
-  const BrickRec* __restrict__ bricks;
-  SlotRec* slots;
-  float* U;
-  float* H;
-  float* UO;  // reaction outputs (= U, H except when timing the kernel in isolation)
-  float* HO;
-  float* R0;
-  double* DACC;
-  float* USTAR;
-  uint8_t* VF;
-  const float* ALPHA;
-  int32_t n_bricks;
-  int32_t xcd;
-  int32_t diag;
-};
-
-enum { MODE_MID = 0, MODE_FINAL_X = 1, MODE_DACC_SET = 2, MODE_DACC_ADD = 3, MODE_DACC_ADD_X = 4 };
-
-__device__ __forceinline__ int wave_brick(int xcd) {
-  int blk = blockIdx.x;
-  if (xcd) {
-    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous
-    // run of bricks so halo gathers hit that XCD's L2.  The grid is a multiple of 8.
-    const int per = gridDim.x >> 3;
-    blk = (blk & 7) * per + (blk >> 3);
-  }
-  return __builtin_amdgcn_readfirstlane(blk * kWaves + (int)(threadIdx.x >> 6));
-}
-
-__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
-
-__device__ __forceinline__ int compact_index(const BrickRec& br, int lane) {
-  return br.start + __popcll(br.mask & ((1ULL << lane) - 1ULL));
-}
-
-__device__ __forceinline__ uint8_t sched_now(const SlotRec& S) { return S.sched[S.step]; }
-
-// Effective alpha = dt c / (2 Delta^2) of voxel idx for this step.
-template <typename T>
-__device__ __forceinline__ T alpha_of(const ArenaDev& A, const SlotRec& S, int idx, uint8_t sc,
-                                      bool valid) {
-  T a = sizeof(T) == 8 ? (T)S.alpha64 : (T)S.alpha;
-  if (S.hetero && valid) a = (T)A.ALPHA[idx];
-  if (S.needs_vf && valid) {
-    const uint8_t f = A.VF[idx];
-    if ((f & PHS_VF_SCAR) || ((sc & PHS_SCHED_TEMP_BLOCK) && (f & PHS_VF_TEMP_BLOCK))) a = (T)0;
-  }
-  return a;
-}
-
-// sum over present face neighbours m of (v_m - v_n), v held per lane (0 on empty lanes),
-// cross-brick values gathered from src[] (scaled by sc).  Accumulated in type T.
-template <int BX, int BY, int BZ, typename T, typename V>
-__device__ __forceinline__ T neighbour_sum(const ArenaDev& A, const BrickRec& br, int lane,
-                                           bool valid, V v, const V* __restrict__ src, V sc) {
-  constexpr int B[3] = {BX, BY, BZ};
-  constexpr int St[3] = {1, BX, BX * BY};
-  const int l[3] = {lane % BX, (lane / BX) % BY, lane / (BX * BY)};
-  T s = (T)0;
-#pragma unroll
-  for (int d = 0; d < 6; ++d) {
-    const int ax = d >> 1;
-    const int sg = (d & 1) ? -1 : 1;
-    const bool inb = sg > 0 ? (l[ax] < B[ax] - 1) : (l[ax] > 0);
-    const int partner = inb ? lane + sg * St[ax] : lane;
-    const V pv = __shfl(v, partner);
-    const int nbk = __builtin_amdgcn_readfirstlane(br.nbb[d]);
-    uint64_t nm = 0;
-    int ns = 0;
-    if (nbk >= 0) {
-      nm = A.bricks[nbk].mask;
-      ns = A.bricks[nbk].start;
-    }
-    if (inb) {
-      if ((br.mask >> partner) & 1ULL) s += (T)pv - (T)v;
-    } else if (valid && nbk >= 0) {
-      const int cell = lane - sg * (B[ax] - 1) * St[ax];
-      if ((nm >> cell) & 1ULL) {
-        const V w = src[ns + __popcll(nm & ((1ULL << cell) - 1ULL))] * sc;
-        s += (T)w - (T)v;
-      }
-    }
-  }
-  return s;
-}
-
-__device__ __forceinline__ void rhs32(const SlotRec& S, float u, float h, float J, float& du,
-                                      float& dh) {
-  if (S.model == 0) {
-    du = h * u * u * (1.0f - u) * S.p[2] - u * S.p[3] + J;
-    dh = (u < S.p[4]) ? (1.0f - h) * S.p[0] : -h * S.p[1];
-  } else {
-    const float k = S.p[0], a = S.p[1];
-    du = -k * u * (u - a) * (u - 1.0f) - u * h + J;
-    dh = (S.p[2] + S.p[3] * h * __builtin_amdgcn_rcpf(u + S.p[4])) * (-h - k * u * (u - a - 1.0f));
-  }
-}
-
-__global__ void __launch_bounds__(kBlock) arena_reaction(ArenaDev A) {
-  const int b = wave_brick(A.xcd);
-  if (b >= A.n_bricks) return;
-  const BrickRec& br = A.bricks[b];
-  const int slot = br.slot;
-  if (slot < 0) return;
-  const SlotRec& S = A.slots[slot];
-  if (!S.active || S.step >= S.n_steps) return;
-  const int lane = lane_id();
-  if (!((br.mask >> lane) & 1ULL)) return;
-  const int idx = compact_index(br, lane);
-  const uint8_t sc = sched_now(S);
-  const uint8_t f = A.VF[idx];
-  const float J = (((sc & PHS_SCHED_S1) && (f & PHS_VF_S1)) || ((sc & PHS_SCHED_S2) && (f & PHS_VF_S2)))
-                      ? 1.0f
-                      : 0.0f;
-  const float dt = S.dt;
-  const float u = A.U[idx], h = A.H[idx];
-  float k1u, k1h, k2u, k2h, k3u, k3h, k4u, k4h;
-  rhs32(S, u, h, J, k1u, k1h);
-  rhs32(S, u + 0.5f * dt * k1u, h + 0.5f * dt * k1h, J, k2u, k2h);
-  rhs32(S, u + 0.5f * dt * k2u, h + 0.5f * dt * k2h, J, k3u, k3h);
-  rhs32(S, u + dt * k3u, h + dt * k3h, J, k4u, k4h);
-  const float s6 = dt * (1.0f / 6.0f);
-  A.UO[idx] = u + s6 * (k1u + 2.0f * k2u + 2.0f * k3u + k4u);
-  A.HO[idx] = h + s6 * (k1h + 2.0f * k2h + 2.0f * k3h + k4h);
-}
-
-// r0 = 2 alpha sum (u*_m - u*_n), in FP64 from FP32 u*.
-template <int BX, int BY, int BZ>
-__global__ void __launch_bounds__(kBlock) arena_rhs(ArenaDev A) {
-  const int b = wave_brick(A.xcd);
-  if (b >= A.n_bricks) return;
-  const BrickRec& br = A.bricks[b];
-  const int slot = br.slot;
-  if (slot < 0) return;
-  const SlotRec& S = A.slots[slot];
-  if (!S.active || S.step >= S.n_steps) return;
-  const int lane = lane_id();
-  const bool valid = (br.mask >> lane) & 1ULL;
-  const int idx = compact_index(br, lane);
-  const float un = valid ? A.U[idx] : 0.0f;
-  const double s = neighbour_sum<BX, BY, BZ, double, float>(A, br, lane, valid, un, A.U, 1.0f);
-  if (!valid) return;
-  const double a = alpha_of<double>(A, S, idx, sched_now(S), valid);
-  A.R0[idx] = (float)(2.0 * a * s);
-}
-
-// FP64 residual of the accumulated correction: r = 2 alpha sum(u*_m - u*_n) - A dacc.
-template <int BX, int BY, int BZ>
-__global__ void __launch_bounds__(kBlock) arena_residual(ArenaDev A) {
-  const int b = wave_brick(A.xcd);
-  if (b >= A.n_bricks) return;
-  const BrickRec& br = A.bricks[b];
-  const int slot = br.slot;
-  if (slot < 0) return;
-  const SlotRec& S = A.slots[slot];
-  if (!S.active || S.step >= S.n_steps) return;
-  const int lane = lane_id();
-  const bool valid = (br.mask >> lane) & 1ULL;
-  const int idx = compact_index(br, lane);
-  const float un = valid ? A.U[idx] : 0.0f;
-  const double su = neighbour_sum<BX, BY, BZ, double, float>(A, br, lane, valid, un, A.U, 1.0f);
-  const double dn = valid ? A.DACC[idx] : 0.0;
-  const double sd = neighbour_sum<BX, BY, BZ, double, double>(A, br, lane, valid, dn, A.DACC, 1.0);
-  if (!valid) return;
-  const double a = alpha_of<double>(A, S, idx, sched_now(S), valid);
-  A.R0[idx] = (float)(2.0 * a * su - (dn - a * sd));
-}
-
-// Activation time, samples, and state write of the final pass.
-__device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotRec& S, int idx, float x) {
-  A.U[idx] = x;
-  const uint8_t f = A.VF[idx];
-  const bool now = x >= S.lat_thr;
-  const bool above = (f & PHS_VF_ABOVE) != 0;
-  const int64_t local = (int64_t)idx - S.voxel_off;
-  if (now && !above) {
-    int bt = (f & PHS_VF_BEAT_MASK) >> PHS_VF_BEAT_SHIFT;
-    if (bt < S.max_beats && S.lat) S.lat[(int64_t)bt * S.n_voxel + local] = (int32_t)S.step;
-    bt = bt < 7 ? bt + 1 : 7;
-    A.VF[idx] = (uint8_t)((f & 0x0F) | (bt << PHS_VF_BEAT_SHIFT) | PHS_VF_ABOVE);
-  } else if (!now && above) {
-    A.VF[idx] = (uint8_t)(f & ~PHS_VF_ABOVE);
-  }
-  int64_t row = -1;
-  if (S.nspm > 0) {
-    if (S.step % S.nspm == 0) row = S.step / S.nspm;
-  } else {
-    row = S.step;
-  }
-  if (row >= 0 && row < S.n_samples) {
-    const int64_t o = row * S.n_voxel + local;
-    if (S.su) S.su[o] = x;
-    if (S.sh) S.sh[o] = A.H[idx];
-  }
-}
-
-// One Chebyshev sweep d_{k+1} = a_k d_k + b_k d_{k-1} + g_k (r0 - A d_k).
-// cur_kind 1: d_k = r0 / theta (k = 1).  prev_kind 0: d_{k-1} = 0; 1: r0 / theta; 2: prev[].
-template <int BX, int BY, int BZ, int MODE>
-__global__ void __launch_bounds__(kBlock) arena_sweep(ArenaDev A, int k, const float* cur,
-                                                      int cur_kind, const float* prev,
-                                                      int prev_kind, float* out) {
-  const int b = wave_brick(A.xcd);
-  if (b >= A.n_bricks) return;
-  const BrickRec& br = A.bricks[b];
-  const int slot = br.slot;
-  if (slot < 0) return;
-  const SlotRec& S = A.slots[slot];
-  if (!S.active || S.step >= S.n_steps) return;
-  const int lane = lane_id();
-  const bool valid = (br.mask >> lane) & 1ULL;
-  const int idx = compact_index(br, lane);
-  const float inv_theta = S.coef[0][2];
-  const float* src = cur_kind ? A.R0 : cur;
-  const float sc = cur_kind ? inv_theta : 1.0f;
-  const float dn = valid ? src[idx] * sc : 0.0f;
-  const float s = neighbour_sum<BX, BY, BZ, float, float>(A, br, lane, valid, dn, src, sc);
-  if (!valid) return;
-  const uint8_t schedb = sched_now(S);
-  const float a = alpha_of<float>(A, S, idx, schedb, valid);
-  const float r0 = A.R0[idx];
-  const float Ad = dn - a * s;
-  const float dp = prev_kind == 0 ? 0.0f : (prev_kind == 1 ? r0 * inv_theta : prev[idx]);
-  const float dnew = S.coef[k][0] * dn + S.coef[k][1] * dp + S.coef[k][2] * (r0 - Ad);
-  if (MODE == MODE_MID) {
-    out[idx] = dnew;
-  } else if (MODE == MODE_FINAL_X) {
-    const float us = A.U[idx];
-    if (A.diag) {
-      A.USTAR[idx] = us;
-      A.DACC[idx] = (double)dnew;
-    }
-    finish_voxel(A, S, idx, us + dnew);
-  } else if (MODE == MODE_DACC_SET) {
-    A.DACC[idx] = (double)dnew;
-  } else if (MODE == MODE_DACC_ADD) {
-    A.DACC[idx] += (double)dnew;
-  } else {
-    const double acc = A.DACC[idx] + (double)dnew;
-    const float us = A.U[idx];
-    A.DACC[idx] = acc;
-    if (A.diag) A.USTAR[idx] = us;
-    finish_voxel(A, S, idx, (float)((double)us + acc));
-  }
-}
-
-__global__ void arena_advance(SlotRec* slots, int n) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < n && slots[s].active && slots[s].step < slots[s].n_steps) slots[s].step += 1;
-}
-
-__global__ void arena_deactivate(SlotRec* slots, int slot) { slots[slot].active = 0; }
-
-__global__ void arena_invalidate(BrickRec* bricks, int64_t b0, int64_t nb) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nb) bricks[b0 + i].slot = -1;
-}
-
-// Diagnostic: sum of r^2 and b^2 for b = u* + alpha sum(u*_m - u*_n),
-// x = u* + dacc, r = b - (x - alpha sum(x_m - x_n)), at the last completed step.
-template <int BX, int BY, int BZ>
-__global__ void __launch_bounds__(kBlock) arena_diag(ArenaDev A, int slot_sel, double* acc) {
-  const int b = wave_brick(A.xcd);
-  if (b >= A.n_bricks) return;
-  const BrickRec& br = A.bricks[b];
-  if (br.slot != slot_sel) return;
-  const SlotRec& S = A.slots[br.slot];
-  const int lane = lane_id();
-  const bool valid = (br.mask >> lane) & 1ULL;
-  const int idx = compact_index(br, lane);
-  const float us = valid ? A.USTAR[idx] : 0.0f;
-  const double su = neighbour_sum<BX, BY, BZ, double, float>(A, br, lane, valid, us, A.USTAR, 1.0f);
-  const double dn = valid ? A.DACC[idx] : 0.0;
-  const double sd = neighbour_sum<BX, BY, BZ, double, double>(A, br, lane, valid, dn, A.DACC, 1.0);
-  double r2 = 0.0, b2 = 0.0;
-  if (valid) {
-    const uint8_t sc = S.step > 0 ? S.sched[S.step - 1] : 0;
-    double a = S.hetero ? (double)A.ALPHA[idx] : S.alpha64;
-    if (S.needs_vf) {
-      const uint8_t f = A.VF[idx];
-      if ((f & PHS_VF_SCAR) || ((sc & PHS_SCHED_TEMP_BLOCK) && (f & PHS_VF_TEMP_BLOCK))) a = 0.0;
-    }
-    const double bb = (double)us + a * su;
-    // x = u* + d: A x = u* + d - alpha (su + sd)
-    const double r = bb - ((double)us + dn - a * (su + sd));
-    r2 = r * r;
-    b2 = bb * bb;
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    r2 += __shfl_xor(r2, o);
-    b2 += __shfl_xor(b2, o);
-  }
-  if (lane == 0) {
-    atomicAdd(acc, r2);
-    atomicAdd(acc + 1, b2);
-  }
-}
-
-__global__ void scatter_rows_f32(float* __restrict__ dst, const float* __restrict__ src,
-                                 const int32_t* __restrict__ perm, int64_t rows, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t r = blockIdx.y;
-  for (int64_t rr = r; rr < rows; rr += gridDim.y)
-    if (i < n) dst[rr * n + perm[i]] = src[rr * n + i];
-}
-
-__global__ void scatter_rows_i32(int32_t* __restrict__ dst, const int32_t* __restrict__ src,
-                                 const int32_t* __restrict__ perm, int64_t rows, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (int64_t rr = blockIdx.y; rr < rows; rr += gridDim.y)
-    if (i < n) dst[rr * n + perm[i]] = src[rr * n + i];
-}
 
 // ------------------------------------------------------------------ host side
 class RangeAlloc {
@@ -450,7 +103,7 @@ struct phs_arena {
   BrickRec* bricks = nullptr;
   SlotRec* slots = nullptr;
   float *U = nullptr, *H = nullptr, *R0 = nullptr, *DA = nullptr, *DB = nullptr,
-        *USTAR = nullptr, *ALPHA = nullptr;
+        *USTAR = nullptr, *ALPHA = nullptr, *D1 = nullptr, *D2 = nullptr;
   double* DACC = nullptr;
   uint8_t* VF = nullptr;
   double* diag_acc = nullptr;
@@ -479,76 +132,119 @@ ArenaDev dev_view(const phs_arena* a) {
   A.USTAR = a->USTAR;
   A.VF = a->VF;
   A.ALPHA = a->ALPHA;
+  A.D1 = a->D1;
+  A.D2 = a->D2;
   A.n_bricks = (int32_t)a->brick_high;
   A.xcd = a->cfg.xcd_remap ? 1 : 0;
   A.diag = a->cfg.diag ? 1 : 0;
   return A;
 }
 
-int grid_for(const phs_arena* a) {
+int nb_per_wave(const phs_arena* a) { return a->cfg.bricks_per_wave == 2 ? 2 : 1; }
+
+int grid_for_nb1(const phs_arena* a) {
   int g = (int)((a->brick_high + kWaves - 1) / kWaves);
   if (g < 1) g = 1;
   if (a->cfg.xcd_remap) g = (g + 7) / 8 * 8;
   return g;
 }
 
-template <int BX, int BY, int BZ>
-void launch_step_shape(phs_arena* a, hipStream_t s) {
-  const ArenaDev A = dev_view(a);
-  const int grid = grid_for(a);
-  const phs_arena_config& c = a->cfg;
-  arena_reaction<<<grid, kBlock, 0, s>>>(A);
-  arena_rhs<BX, BY, BZ><<<grid, kBlock, 0, s>>>(A);
-  float* bufs[2] = {a->DA, a->DB};
-  // One Chebyshev solve of m sweeps on the right-hand side in R0; `last` selects the final
-  // pass behaviour.
-  auto solve = [&](int m, int last_mode) {
-    for (int k = 1; k < m; ++k) {
-      const float* cur = k == 1 ? nullptr : bufs[k % 2];
-      const int cur_kind = k == 1 ? 1 : 0;
-      const float* prev = k <= 2 ? nullptr : bufs[(k - 1) % 2];
-      const int prev_kind = k == 1 ? 0 : (k == 2 ? 1 : 2);
-      float* out = bufs[(k + 1) % 2];
-      if (k < m - 1) {
-        arena_sweep<BX, BY, BZ, MODE_MID><<<grid, kBlock, 0, s>>>(A, k, cur, cur_kind, prev,
-                                                                    prev_kind, out);
-      } else {
-        switch (last_mode) {
-          case MODE_FINAL_X:
-            arena_sweep<BX, BY, BZ, MODE_FINAL_X><<<grid, kBlock, 0, s>>>(A, k, cur, cur_kind, prev,
-                                                                            prev_kind, out);
-            break;
-          case MODE_DACC_SET:
-            arena_sweep<BX, BY, BZ, MODE_DACC_SET><<<grid, kBlock, 0, s>>>(A, k, cur, cur_kind,
-                                                                             prev, prev_kind, out);
-            break;
-          case MODE_DACC_ADD:
-            arena_sweep<BX, BY, BZ, MODE_DACC_ADD><<<grid, kBlock, 0, s>>>(A, k, cur, cur_kind,
-                                                                             prev, prev_kind, out);
-            break;
-          default:
-            arena_sweep<BX, BY, BZ, MODE_DACC_ADD_X><<<grid, kBlock, 0, s>>>(A, k, cur, cur_kind,
-                                                                               prev, prev_kind, out);
-        }
-      }
-    }
-  };
-  if (c.refine <= 0) {
-    solve(c.sweeps, MODE_FINAL_X);
-  } else {
-    solve(c.sweeps, MODE_DACC_SET);
-    for (int r = 0; r < c.refine; ++r) {
-      arena_residual<BX, BY, BZ><<<grid, kBlock, 0, s>>>(A);
-      solve(c.refine_sweeps, r == c.refine - 1 ? MODE_DACC_ADD_X : MODE_DACC_ADD);
-    }
-  }
-  arena_advance<<<(c.max_sims + 63) / 64, 64, 0, s>>>(a->slots, c.max_sims);
+int grid_for(const phs_arena* a) {
+  const int per_block = kWaves * nb_per_wave(a);
+  int g = (int)((a->brick_high + per_block - 1) / per_block);
+  if (g < 1) g = 1;
+  if (a->cfg.xcd_remap) g = (g + 7) / 8 * 8;
+  return g;
 }
 
-template <int BX, int BY, int BZ>
-void launch_diag_shape(phs_arena* a, int slot, hipStream_t s) {
-  arena_diag<BX, BY, BZ><<<grid_for(a), kBlock, 0, s>>>(dev_view(a), slot, a->diag_acc);
-}
+// Launches of one time step for brick shape (BX, BY, BZ) and NB bricks per wavefront.
+template <int BX, int BY, int BZ, int NB, int GUESS>
+struct Step {
+  static void sweep(int mode, int grid, hipStream_t s, const ArenaDev& A, int k, const float* cur,
+                    int cur_kind, const float* prev, int prev_kind, float* out) {
+    switch (mode) {
+      case MODE_MID:
+        k_sweep<BX, BY, BZ, MODE_MID, NB, GUESS><<<grid, kBlock, 0, s>>>(A,k, cur, cur_kind, prev, prev_kind, out);
+        break;
+      case MODE_FINAL_X:
+        k_sweep<BX, BY, BZ, MODE_FINAL_X, NB, GUESS><<<grid, kBlock, 0, s>>>(A,k, cur, cur_kind, prev, prev_kind, out);
+        break;
+      case MODE_DACC_SET:
+        k_sweep<BX, BY, BZ, MODE_DACC_SET, NB, GUESS><<<grid, kBlock, 0, s>>>(A,k, cur, cur_kind, prev, prev_kind, out);
+        break;
+      case MODE_DACC_ADD:
+        k_sweep<BX, BY, BZ, MODE_DACC_ADD, NB, GUESS><<<grid, kBlock, 0, s>>>(A,k, cur, cur_kind, prev, prev_kind, out);
+        break;
+      default:
+        k_sweep<BX, BY, BZ, MODE_DACC_ADD_X, NB, GUESS><<<grid, kBlock, 0, s>>>(A,k, cur, cur_kind, prev, prev_kind, out);
+    }
+  }
+
+  static void run(phs_arena* a, hipStream_t s) {
+    const ArenaDev A = dev_view(a);
+    const int grid = grid_for(a);
+    const int grid1 = grid_for_nb1(a);
+    const phs_arena_config& c = a->cfg;
+    k_reaction<NB><<<grid, kBlock, 0, s>>>(A);
+    k_rhs<BX, BY, BZ, NB, GUESS><<<grid, kBlock, 0, s>>>(A);
+    float* bufs[2] = {a->DA, a->DB};
+    // One Chebyshev solve of m sweeps on the right-hand side held in R0.
+    auto solve = [&](int m, int last_mode) {
+      for (int k = 1; k < m; ++k) {
+        const float* cur = k == 1 ? nullptr : bufs[k % 2];
+        const int cur_kind = k == 1 ? 1 : 0;
+        const float* prev = k <= 2 ? nullptr : bufs[(k - 1) % 2];
+        const int prev_kind = k == 1 ? 0 : (k == 2 ? 1 : 2);
+        sweep(k < m - 1 ? MODE_MID : last_mode, grid, s, A, k, cur, cur_kind, prev, prev_kind,
+              bufs[(k + 1) % 2]);
+      }
+    };
+    if (c.refine <= 0) {
+      solve(c.sweeps, MODE_FINAL_X);
+    } else {
+      solve(c.sweeps, MODE_DACC_SET);
+      for (int r = 0; r < c.refine; ++r) {
+        k_residual<BX, BY, BZ><<<grid1, kBlock, 0, s>>>(A);
+        solve(c.refine_sweeps, r == c.refine - 1 ? MODE_DACC_ADD_X : MODE_DACC_ADD);
+      }
+    }
+    k_advance<<<(c.max_sims + 63) / 64, 64, 0, s>>>(a->slots, c.max_sims);
+  }
+
+  // Average duration (ms) of the reaction, rhs and sweep kernels over `reps` launches,
+  // timed with HIP events on `s`.  The reaction writes into scratch (DA, DB) and the sweep
+  // runs in MID mode, so no simulation state changes.
+  static hipError_t profile(phs_arena* a, int reps, double* ms, hipStream_t s) {
+    const ArenaDev A = dev_view(a);
+    const int grid = grid_for(a);
+    hipEvent_t ev[4];
+    for (auto& e : ev)
+      if (hipError_t r = hipEventCreate(&e)) return r;
+    ArenaDev R = A;
+    R.UO = a->DA;
+    R.HO = a->DB;
+    hipEventRecord(ev[0], s);
+    for (int i = 0; i < reps; ++i) k_reaction<NB><<<grid, kBlock, 0, s>>>(R);
+    hipEventRecord(ev[1], s);
+    for (int i = 0; i < reps; ++i) k_rhs<BX, BY, BZ, NB, GUESS><<<grid, kBlock, 0, s>>>(A);
+    hipEventRecord(ev[2], s);
+    for (int i = 0; i < reps; ++i)
+      k_sweep<BX, BY, BZ, MODE_MID, NB, GUESS><<<grid, kBlock, 0, s>>>(A,3, a->DA, 0, a->DB, 2, a->DB);
+    hipEventRecord(ev[3], s);
+    hipError_t r = hipEventSynchronize(ev[3]);
+    for (int i = 0; i < 3; ++i) {
+      float t = 0.0f;
+      hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+      ms[i] = (double)t / reps;
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    return r;
+  }
+
+  static void diag(phs_arena* a, int slot, hipStream_t s) {
+    k_diag<BX, BY, BZ><<<grid_for_nb1(a), kBlock, 0, s>>>(dev_view(a), slot, a->diag_acc);
+  }
+};
 
 bool shape_supported(const int* b) {
   const int s[][3] = {{4, 4, 4}, {8, 8, 1}, {8, 4, 2}, {4, 8, 2}, {16, 4, 1}};
@@ -557,62 +253,44 @@ bool shape_supported(const int* b) {
   return false;
 }
 
-void launch_step(phs_arena* a, hipStream_t s) {
-  const int* b = a->cfg.brick;
-  if (b[0] == 8 && b[1] == 8) launch_step_shape<8, 8, 1>(a, s);
-  else if (b[0] == 8 && b[1] == 4) launch_step_shape<8, 4, 2>(a, s);
-  else if (b[0] == 4 && b[1] == 8) launch_step_shape<4, 8, 2>(a, s);
-  else if (b[0] == 16) launch_step_shape<16, 4, 1>(a, s);
-  else launch_step_shape<4, 4, 4>(a, s);
+template <int BX, int BY, int BZ, typename F>
+void with_nb(const phs_arena* a, F&& f) {
+  switch (nb_per_wave(a)) {
+    case 2:
+      if (a->cfg.guess == 2) f(Step<BX, BY, BZ, 2, 2>());
+      else if (a->cfg.guess == 1) f(Step<BX, BY, BZ, 2, 1>());
+      else f(Step<BX, BY, BZ, 2, 0>());
+      break;
+    default:
+      if (a->cfg.guess == 2) f(Step<BX, BY, BZ, 1, 2>());
+      else if (a->cfg.guess == 1) f(Step<BX, BY, BZ, 1, 1>());
+      else f(Step<BX, BY, BZ, 1, 0>());
+  }
 }
 
-// Average duration (ms) of each kernel class over `reps` launches, timed with HIP events
-// on `s`.  The reaction writes into scratch (DA, DB) and the sweep runs in MID mode, so no
-// simulation state changes.
-template <int BX, int BY, int BZ>
-hipError_t profile_shape(phs_arena* a, int reps, double* ms, hipStream_t s) {
-  ArenaDev A = dev_view(a);
-  const int grid = grid_for(a);
-  hipEvent_t ev[4];
-  for (auto& e : ev)
-    if (hipError_t r = hipEventCreate(&e)) return r;
-  ArenaDev R = A;
-  R.UO = a->DA;
-  R.HO = a->DB;
-  hipEventRecord(ev[0], s);
-  for (int i = 0; i < reps; ++i) arena_reaction<<<grid, kBlock, 0, s>>>(R);
-  hipEventRecord(ev[1], s);
-  for (int i = 0; i < reps; ++i) arena_rhs<BX, BY, BZ><<<grid, kBlock, 0, s>>>(A);
-  hipEventRecord(ev[2], s);
-  for (int i = 0; i < reps; ++i)
-    arena_sweep<BX, BY, BZ, MODE_MID><<<grid, kBlock, 0, s>>>(A, 3, a->DA, 0, a->DB, 2, a->DB);
-  hipEventRecord(ev[3], s);
-  hipError_t r = hipEventSynchronize(ev[3]);
-  for (int i = 0; i < 3; ++i) {
-    float t = 0.0f;
-    hipEventElapsedTime(&t, ev[i], ev[i + 1]);
-    ms[i] = (double)t / reps;
-  }
-  for (auto& e : ev) hipEventDestroy(e);
-  return r;
+// Calls f(Step<shape, NB>()) for the arena's brick shape and bricks per wavefront.
+template <typename F>
+void dispatch(const phs_arena* a, F&& f) {
+  const int* b = a->cfg.brick;
+  if (b[0] == 8 && b[1] == 8) with_nb<8, 8, 1>(a, f);
+  else if (b[0] == 8 && b[1] == 4) with_nb<8, 4, 2>(a, f);
+  else if (b[0] == 4 && b[1] == 8) with_nb<4, 8, 2>(a, f);
+  else if (b[0] == 16) with_nb<16, 4, 1>(a, f);
+  else with_nb<4, 4, 4>(a, f);
+}
+
+void launch_step(phs_arena* a, hipStream_t s) {
+  dispatch(a, [&](auto st) { decltype(st)::run(a, s); });
 }
 
 hipError_t profile_kernels(phs_arena* a, int reps, double* ms, hipStream_t s) {
-  const int* b = a->cfg.brick;
-  if (b[0] == 8 && b[1] == 8) return profile_shape<8, 8, 1>(a, reps, ms, s);
-  if (b[0] == 8 && b[1] == 4) return profile_shape<8, 4, 2>(a, reps, ms, s);
-  if (b[0] == 4 && b[1] == 8) return profile_shape<4, 8, 2>(a, reps, ms, s);
-  if (b[0] == 16) return profile_shape<16, 4, 1>(a, reps, ms, s);
-  return profile_shape<4, 4, 4>(a, reps, ms, s);
+  hipError_t r = hipSuccess;
+  dispatch(a, [&](auto st) { r = decltype(st)::profile(a, reps, ms, s); });
+  return r;
 }
 
 void launch_diag(phs_arena* a, int slot, hipStream_t s) {
-  const int* b = a->cfg.brick;
-  if (b[0] == 8 && b[1] == 8) launch_diag_shape<8, 8, 1>(a, slot, s);
-  else if (b[0] == 8 && b[1] == 4) launch_diag_shape<8, 4, 2>(a, slot, s);
-  else if (b[0] == 4 && b[1] == 8) launch_diag_shape<4, 8, 2>(a, slot, s);
-  else if (b[0] == 16) launch_diag_shape<16, 4, 1>(a, slot, s);
-  else launch_diag_shape<4, 4, 4>(a, slot, s);
+  dispatch(a, [&](auto st) { decltype(st)::diag(a, slot, s); });
 }
 
 void free_slot_host(SlotHost& h) {
@@ -670,6 +348,9 @@ int phs_arena_create(const phs_arena_config* cfg, phs_arena** handle) {
   if (cfg->sweeps < 2 || cfg->sweeps > kMaxSweeps)
     return phs::fail(PHS_ERR_ARG, "sweeps must be in [2, 64]");
   if (cfg->refine < 0 || cfg->refine > 4) return phs::fail(PHS_ERR_ARG, "refine must be in [0, 4]");
+  if (cfg->bricks_per_wave != 0 && cfg->bricks_per_wave != 1 && cfg->bricks_per_wave != 2)
+    return phs::fail(PHS_ERR_ARG, "bricks_per_wave must be 1 or 2");
+  if (cfg->guess < 0 || cfg->guess > 2) return phs::fail(PHS_ERR_ARG, "guess must be 0, 1 or 2");
   if (cfg->refine > 0 && (cfg->refine_sweeps < 2 || cfg->refine_sweeps > kMaxSweeps))
     return phs::fail(PHS_ERR_ARG, "refine_sweeps must be in [2, 64]");
   phs_arena* a = new phs_arena();
@@ -684,6 +365,8 @@ int phs_arena_create(const phs_arena_config* cfg, phs_arena** handle) {
       (e = hipMalloc(&a->DB, 4 * nv)) || (e = hipMalloc(&a->VF, nv)) ||
       (e = hipMalloc(&a->ALPHA, 4 * nv)) || (e = hipMalloc(&a->diag_acc, 16)) ||
       (need_dacc && (e = hipMalloc(&a->DACC, 8 * nv))) ||
+      (cfg->guess >= 1 && (e = hipMalloc(&a->D1, 4 * nv))) ||
+      (cfg->guess == 2 && (e = hipMalloc(&a->D2, 4 * nv))) ||
       (cfg->diag && (e = hipMalloc(&a->USTAR, 4 * nv)))) {
     phs_arena_destroy(a);
     return phs::fail(PHS_ERR_OOM, std::string("arena hipMalloc: ") + hipGetErrorString(e));
@@ -779,7 +462,8 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
     r.slot = slot;
     for (int k = 0; k < 6; ++k) {
       const int32_t nb = P.nbb[(size_t)b * 6 + k];
-      r.nbb[k] = nb >= 0 ? (int32_t)(bo + nb) : -1;
+      r.nmask[k] = nb >= 0 ? P.mask[nb] : 0;
+      r.nstart[k] = nb >= 0 ? (int32_t)(vo + P.start[nb]) : 0;
     }
     r.pad[0] = r.pad[1] = 0;
     hb[b] = r;
@@ -837,12 +521,14 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
   }
   sr.step = 0;
   sr.n_steps = d->n_steps;
+  sr.cur_sched = d->n_steps > 0 ? d->schedule[0] : 0;
+  sr.cur_row = d->n_samples > 0 ? 0 : -1;
   sr.voxel_off = vo;
   sr.su = out ? (float*)out->samples_u : nullptr;
   sr.sh = out ? (float*)out->samples_h : nullptr;
   sr.lat = out ? out->lat : nullptr;
   sr.n_voxel = N;
-  sr.active = 1;
+  sr.live = d->n_steps > 0 ? 1 : 0;
   sr.model = d->model;
   sr.nspm = d->steps_per_sample;
   sr.n_samples = d->n_samples;
@@ -878,6 +564,8 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
        (e = hipMemcpyAsync(h.sched_dev, hsched, (size_t)d->n_steps, hipMemcpyHostToDevice, s))) ||
       (out && out->lat && out->max_beats > 0 &&
        (e = hipMemsetAsync(out->lat, 0xFF, 4 * (size_t)out->max_beats * N, s))) ||
+      (a->D1 && (e = hipMemsetAsync(a->D1 + vo, 0, sz_f, s))) ||
+      (a->D2 && (e = hipMemsetAsync(a->D2 + vo, 0, sz_f, s))) ||
       (e = hipMemcpyAsync(a->slots + slot, hs, sz_sl, hipMemcpyHostToDevice, s)) ||
       (e = hipEventCreateWithFlags(&h.upload_done, hipEventDisableTiming)) ||
       (e = hipEventRecord(h.upload_done, s)))
@@ -895,8 +583,8 @@ int phs_arena_remove(phs_arena* a, int32_t slot, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   // Stream-ordered: steps already queued on `stream` still see the simulation; later work
   // (including an add that reuses its ranges) does not.
-  arena_invalidate<<<(h.n_bricks + 255) / 256, 256, 0, s>>>(a->bricks, h.brick_off, h.n_bricks);
-  arena_deactivate<<<1, 1, 0, s>>>(a->slots, slot);
+  k_invalidate<<<(h.n_bricks + 255) / 256, 256, 0, s>>>(a->bricks, h.brick_off, h.n_bricks);
+  k_deactivate<<<1, 1, 0, s>>>(a->slots, slot);
   PHS_HIP_CHECK(hipGetLastError());
   DeferredFree g;
   PHS_HIP_CHECK(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
@@ -979,8 +667,8 @@ int phs_arena_state(phs_arena* a, int32_t slot, float* u, float* h, void* stream
   const SlotHost& sl = a->sh[slot];
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid((sl.n_voxel + 255) / 256, 1);
-  if (u) scatter_rows_f32<<<grid, 256, 0, s>>>(u, a->U + sl.voxel_off, sl.perm_dev, 1, sl.n_voxel);
-  if (h) scatter_rows_f32<<<grid, 256, 0, s>>>(h, a->H + sl.voxel_off, sl.perm_dev, 1, sl.n_voxel);
+  if (u) k_scatter_f32<<<grid, 256, 0, s>>>(u, a->U + sl.voxel_off, sl.perm_dev, 1, sl.n_voxel);
+  if (h) k_scatter_f32<<<grid, 256, 0, s>>>(h, a->H + sl.voxel_off, sl.perm_dev, 1, sl.n_voxel);
   PHS_HIP_CHECK(hipGetLastError());
   return PHS_OK;
 }
@@ -1030,7 +718,7 @@ int phs_arena_counts(phs_arena* a, int64_t* voxels, int64_t* bricks, int64_t* vf
   if (vf_voxels) *vf_voxels = 0;
   for (auto& h : a->sh)
     if (h.used && h.needs_vf && vf_voxels) *vf_voxels += h.n_voxel;
-  if (launch_bricks) *launch_bricks = (int64_t)grid_for(a) * kWaves;
+  if (launch_bricks) *launch_bricks = (int64_t)grid_for(a) * kWaves * nb_per_wave(a);
   return PHS_OK;
 }
 
@@ -1051,6 +739,8 @@ int phs_arena_destroy(phs_arena* a) {
   hipFree(a->DB);
   hipFree(a->USTAR);
   hipFree(a->ALPHA);
+  hipFree(a->D1);
+  hipFree(a->D2);
   hipFree(a->DACC);
   hipFree(a->VF);
   hipFree(a->diag_acc);
@@ -1063,7 +753,7 @@ int phs_scatter_rows_f32(float* dst, const float* src, const int32_t* perm, int6
   if (!dst || !src || !perm || rows < 0 || n < 0) return phs::fail(PHS_ERR_ARG, "bad arguments");
   if (rows == 0 || n == 0) return PHS_OK;
   const dim3 grid((unsigned)((n + 255) / 256), (unsigned)std::min<int64_t>(rows, 65535));
-  scatter_rows_f32<<<grid, 256, 0, (hipStream_t)stream>>>(dst, src, perm, rows, n);
+  k_scatter_f32<<<grid, 256, 0, (hipStream_t)stream>>>(dst, src, perm, rows, n);
   PHS_HIP_CHECK(hipGetLastError());
   return PHS_OK;
 }
@@ -1073,7 +763,7 @@ int phs_scatter_rows_i32(int32_t* dst, const int32_t* src, const int32_t* perm, 
   if (!dst || !src || !perm || rows < 0 || n < 0) return phs::fail(PHS_ERR_ARG, "bad arguments");
   if (rows == 0 || n == 0) return PHS_OK;
   const dim3 grid((unsigned)((n + 255) / 256), (unsigned)std::min<int64_t>(rows, 65535));
-  scatter_rows_i32<<<grid, 256, 0, (hipStream_t)stream>>>(dst, src, perm, rows, n);
+  k_scatter_i32<<<grid, 256, 0, (hipStream_t)stream>>>(dst, src, perm, rows, n);
   PHS_HIP_CHECK(hipGetLastError());
   return PHS_OK;
 }

@@ -229,12 +229,14 @@ class PlainSim:
 # --------------------------------------------------------------------- bricked FP32 arena
 @dataclass
 class SolverConfig:
-    sweeps: int = 16
+    sweeps: int = 6
     refine: int = 0
     refine_sweeps: int = 8
     brick: tuple = (4, 4, 4)
     xcd_remap: bool = True
     diag: bool = False
+    bricks_per_wave: int = 1
+    guess: int = 1
 
 
 @dataclass
@@ -266,6 +268,8 @@ class Arena:
         cfg.refine_sweeps = solver.refine_sweeps
         cfg.xcd_remap = 1 if solver.xcd_remap else 0
         cfg.diag = 1 if solver.diag else 0
+        cfg.bricks_per_wave = int(solver.bricks_per_wave)
+        cfg.guess = int(solver.guess)
         self.cfg = cfg
         self.handle = C.c_void_p()
         _lib.check(self.lib.phs_arena_create(C.byref(cfg), C.byref(self.handle)), "phs_arena_create")

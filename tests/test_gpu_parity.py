@@ -73,7 +73,7 @@ def test_fp32_arena_lat_within_one_step(cuda):
     w = small_shell(t_final=40.0)
     ref = run_oracle(w, max_beats=7)
     prep = _prep(w)
-    arena = runtime.Arena(prep.n_voxel, 1, runtime.SolverConfig(sweeps=16))
+    arena = runtime.Arena(prep.n_voxel, 1, runtime.SolverConfig())
     slot = arena.add(prep)
     arena.step(prep.n_steps)
     su, _, lat = arena.outputs(slot)
