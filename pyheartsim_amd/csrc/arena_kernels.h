@@ -140,9 +140,11 @@ struct Stencil {
   int idx;          // compact index (the brick's first voxel on empty lanes)
   bool valid;
 
+  // `br` is a register copy of the brick record (all of it loaded before any use).
   __device__ __forceinline__ void init(const BrickRec& br) {
     valid = lane_bit(br.mask);
-    idx = br.start + (valid ? mbcnt64(br.mask) : 0);
+    const int own = mbcnt64(br.mask);
+    idx = br.start + (valid ? own : 0);
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
       const int st = SH::step(d), k = SH::wrap(d);
@@ -153,7 +155,8 @@ struct Stencil {
       const uint64_t sh = (d & 1) ? (nm >> k) : (nm << k);
       pcr[d] = sh & ~IN & br.mask;
       const int below = (d & 1) ? __popcll(nm & ((1ULL << k) - 1ULL)) : 0;
-      gaddr[d] = lane_bit(pcr[d]) ? br.nstart[d] + mbcnt64(sh) + below : idx;
+      const int cross = br.nstart[d] + below + mbcnt64(sh);
+      gaddr[d] = lane_bit(pcr[d]) ? cross : idx;
     }
   }
 
@@ -164,20 +167,56 @@ struct Stencil {
   }
 
   // sum over present face neighbours of (v_m - v_n) in type T; own value v, cross values g
-  // scaled by sc.
+  // scaled by sc.  All six lane exchanges are issued before any is consumed.
   template <typename T, typename V>
   __device__ __forceinline__ T sum(V v, const V (&g)[6], V sc, int lane) const {
-    T s = (T)0;
+    V pv[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
       const int st = SH::step(d);
-      const V pv = __shfl(v, ((d & 1) ? lane - st : lane + st) & 63);
-      const V m = lane_bit(pin[d]) ? pv : (lane_bit(pcr[d]) ? g[d] * sc : v);
+      pv[d] = __shfl(v, ((d & 1) ? lane - st : lane + st) & 63);
+    }
+    T s = (T)0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      const V cr = g[d] * sc;
+      const V m = lane_bit(pin[d]) ? pv[d] : (lane_bit(pcr[d]) ? cr : v);
       s += (T)m - (T)v;
     }
     return s;
   }
 };
+
+// Wave-uniform slot values a stencil kernel needs, loaded together.
+struct SlotView {
+  int live, needs_vf, hetero, cur_sched;
+  float alpha, inv_theta, ca, cb, cg;
+  double alpha64;
+  __device__ __forceinline__ void load(const SlotRec& S, int k) {
+    live = S.live;
+    needs_vf = S.needs_vf;
+    hetero = S.hetero;
+    cur_sched = S.cur_sched;
+    alpha = S.alpha;
+    alpha64 = S.alpha64;
+    inv_theta = S.coef[0][2];
+    ca = S.coef[k][0];
+    cb = S.coef[k][1];
+    cg = S.coef[k][2];
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ T alpha_at(const ArenaDev& A, const SlotView& V, int idx) {
+  T a = sizeof(T) == 8 ? (T)V.alpha64 : (T)V.alpha;
+  if (V.hetero) a = (T)A.ALPHA[idx];
+  if (V.needs_vf) {
+    const uint8_t f = A.VF[idx];
+    if ((f & PHS_VF_SCAR) || ((V.cur_sched & PHS_SCHED_TEMP_BLOCK) && (f & PHS_VF_TEMP_BLOCK)))
+      a = (T)0;
+  }
+  return a;
+}
 
 template <typename T>
 __device__ __forceinline__ T alpha_of(const ArenaDev& A, const SlotRec& S, int idx) {
@@ -256,15 +295,22 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
   const int b0 = first_brick<NB>(A.xcd);
   const int lane = lane_id();
   Stencil<BX, BY, BZ> st[NB];
+  SlotView sv[NB];
   bool on[NB];
   float un[NB], d1[NB], d2[NB];
   float gu[NB][6], g1[NB][6], g2[NB][6];
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int b = b0 + j;
-    on[j] = b < A.n_bricks && A.bricks[b].slot >= 0;
+    BrickRec br;
+    on[j] = false;
+    if (b < A.n_bricks) {
+      br = A.bricks[b];
+      on[j] = br.slot >= 0;
+    }
     if (on[j]) {
-      st[j].init(A.bricks[b]);
+      st[j].init(br);
+      sv[j].load(A.slots[br.slot], 0);
       un[j] = A.U[st[j].idx];
       st[j].gather(A.U, gu[j]);
       if (GUESS >= 1) {
@@ -291,9 +337,8 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
       sg = st[j].template sum<double>(gv, q, 1.0f, lane);
       gn = gv;
     }
-    const SlotRec& S = A.slots[A.bricks[b0 + j].slot];
-    if (!st[j].valid || !S.live) continue;
-    const double a = alpha_of<double>(A, S, st[j].idx);
+    if (!st[j].valid || !sv[j].live) continue;
+    const double a = alpha_at<double>(A, sv[j], st[j].idx);
     A.R0[st[j].idx] = (float)(2.0 * a * s - (gn - a * sg));
   }
 }
@@ -331,15 +376,22 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, const float
   const int lane = lane_id();
   const float* src = cur_kind ? A.R0 : cur;
   Stencil<BX, BY, BZ> st[NB];
+  SlotView sv[NB];
   bool on[NB];
   float dn[NB], r0[NB], dp[NB];
   float g6[NB][6];
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int b = b0 + j;
-    on[j] = b < A.n_bricks && A.bricks[b].slot >= 0;
+    BrickRec br;
+    on[j] = false;
+    if (b < A.n_bricks) {
+      br = A.bricks[b];
+      on[j] = br.slot >= 0;
+    }
     if (on[j]) {
-      st[j].init(A.bricks[b]);
+      st[j].init(br);
+      sv[j].load(A.slots[br.slot], k);
       const int i = st[j].idx;
       dn[j] = src[i];
       r0[j] = A.R0[i];
@@ -350,17 +402,17 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, const float
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     if (!on[j]) continue;
-    const SlotRec& S = A.slots[A.bricks[b0 + j].slot];
-    const float inv_theta = S.coef[0][2];
+    const float inv_theta = sv[j].inv_theta;
     const float sc = cur_kind ? inv_theta : 1.0f;
     const float d = st[j].valid ? dn[j] * sc : 0.0f;
     const float s = st[j].template sum<float>(d, g6[j], sc, lane);
-    if (!st[j].valid || !S.live) continue;
+    if (!st[j].valid || !sv[j].live) continue;
     const int i = st[j].idx;
-    const float a = alpha_of<float>(A, S, i);
+    const SlotRec& S = A.slots[A.bricks[b0 + j].slot];
+    const float a = alpha_at<float>(A, sv[j], i);
     const float Ad = d - a * s;
     const float dpv = prev_kind == 1 ? r0[j] * inv_theta : dp[j];
-    const float dnew = S.coef[k][0] * d + S.coef[k][1] * dpv + S.coef[k][2] * (r0[j] - Ad);
+    const float dnew = sv[j].ca * d + sv[j].cb * dpv + sv[j].cg * (r0[j] - Ad);
     if (MODE == MODE_MID) {
       out[i] = dnew;
     } else if (MODE == MODE_FINAL_X) {
