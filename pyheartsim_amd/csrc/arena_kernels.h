@@ -28,7 +28,8 @@ struct BrickRec {
   int32_t start;       // compact index of the first occupied cell
   int32_t slot;        // simulation slot, -1 = unused brick
   uint64_t nmask[6];   // face-neighbour bricks' masks (0 = no brick)
-  int32_t nstart[6];   // face-neighbour bricks' starts
+  int32_t nstart[6];   // gather base of each face-neighbour brick: its start, plus, for a
+                       // - face, the number of its cells below the face-adjacent offset
   int32_t pad[2];
 };
 static_assert(sizeof(BrickRec) == 96, "brick record layout");
@@ -87,6 +88,17 @@ __device__ __forceinline__ float guess_of(int guess, float d1, float d2) {
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// Element access through a 32-bit byte offset from a wave-uniform base, so that the load or
+// store uses the scalar-base addressing form (arena arrays stay below 2^32 bytes).
+template <typename T>
+__device__ __forceinline__ T ld(const T* base, int i) {
+  return *(const T*)((const char*)base + (uint32_t)i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void stv(T* base, int i, T v) {
+  *(T*)((char*)base + (uint32_t)i * (uint32_t)sizeof(T)) = v;
+}
 
 __device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 
@@ -150,12 +162,11 @@ struct Stencil {
       const int st = SH::step(d), k = SH::wrap(d);
       const uint64_t IN = SH::inb(d);
       pin[d] = ((d & 1) ? (br.mask << st) : (br.mask >> st)) & IN & br.mask;
-      const uint64_t nm = br.nmask[d];
-      // the cell of the neighbour brick adjacent to lane l is l - k (+ face) or l + k (- face)
-      const uint64_t sh = (d & 1) ? (nm >> k) : (nm << k);
+      // the cell of the neighbour brick adjacent to lane l is l - k (+ face) or l + k (- face);
+      // nbase already counts the neighbour's cells below k for a - face
+      const uint64_t sh = (d & 1) ? (br.nmask[d] >> k) : (br.nmask[d] << k);
       pcr[d] = sh & ~IN & br.mask;
-      const int below = (d & 1) ? __popcll(nm & ((1ULL << k) - 1ULL)) : 0;
-      const int cross = br.nstart[d] + below + mbcnt64(sh);
+      const int cross = br.nstart[d] + mbcnt64(sh);
       gaddr[d] = lane_bit(pcr[d]) ? cross : idx;
     }
   }
@@ -163,24 +174,51 @@ struct Stencil {
   template <typename V>
   __device__ __forceinline__ void gather(const V* __restrict__ src, V (&g)[6]) const {
 #pragma unroll
-    for (int d = 0; d < 6; ++d) g[d] = src[gaddr[d]];
+    for (int d = 0; d < 6; ++d) g[d] = ld(src, gaddr[d]);
+  }
+
+  // Value of the in-brick neighbour lane in direction D: a DPP row shift when the axis
+  // stays inside one 16-lane row, ds_bpermute otherwise.
+  template <int D, typename V>
+  static __device__ __forceinline__ V exch(V v, int lane) {
+    constexpr int ax = D >> 1;
+    constexpr bool plus = !(D & 1);
+    constexpr int stp = SH::St[ax];
+    if constexpr (SH::B[ax] == 1) {
+      return v;
+    } else {
+      static_assert(sizeof(V) % 4 == 0, "32-bit words");
+      constexpr int W = sizeof(V) / 4;
+      struct Words {
+        int w[W];
+      };
+      Words in = __builtin_bit_cast(Words, v), out;
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        if constexpr (stp * SH::B[ax] <= 16) {
+          out.w[i] = __builtin_amdgcn_mov_dpp(in.w[i], plus ? 0x100 + stp : 0x110 + stp, 0xf, 0xf,
+                                              false);
+        } else {
+          out.w[i] = __builtin_amdgcn_ds_bpermute(((lane + (plus ? stp : -stp)) & 63) << 2, in.w[i]);
+        }
+      }
+      return __builtin_bit_cast(V, out);
+    }
   }
 
   // sum over present face neighbours of (v_m - v_n) in type T; own value v, cross values g
   // scaled by sc.  All six lane exchanges are issued before any is consumed.
   template <typename T, typename V>
   __device__ __forceinline__ T sum(V v, const V (&g)[6], V sc, int lane) const {
-    V pv[6];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) {
-      const int st = SH::step(d);
-      pv[d] = __shfl(v, ((d & 1) ? lane - st : lane + st) & 63);
-    }
+    const V pv[6] = {exch<0>(v, lane), exch<1>(v, lane), exch<2>(v, lane),
+                     exch<3>(v, lane), exch<4>(v, lane), exch<5>(v, lane)};
     T s = (T)0;
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
-      const V cr = g[d] * sc;
-      const V m = lane_bit(pin[d]) ? pv[d] : (lane_bit(pcr[d]) ? cr : v);
+      // two lane-mask selects (pin and pcr are disjoint); absent neighbours give m = v
+      V m = g[d] * sc;
+      m = lane_bit(pcr[d]) ? m : v;
+      m = lane_bit(pin[d]) ? pv[d] : m;
       s += (T)m - (T)v;
     }
     return s;
@@ -209,9 +247,9 @@ struct SlotView {
 template <typename T>
 __device__ __forceinline__ T alpha_at(const ArenaDev& A, const SlotView& V, int idx) {
   T a = sizeof(T) == 8 ? (T)V.alpha64 : (T)V.alpha;
-  if (V.hetero) a = (T)A.ALPHA[idx];
+  if (V.hetero) a = (T)ld(A.ALPHA, idx);
   if (V.needs_vf) {
-    const uint8_t f = A.VF[idx];
+    const uint8_t f = ld(A.VF, idx);
     if ((f & PHS_VF_SCAR) || ((V.cur_sched & PHS_SCHED_TEMP_BLOCK) && (f & PHS_VF_TEMP_BLOCK)))
       a = (T)0;
   }
@@ -221,9 +259,9 @@ __device__ __forceinline__ T alpha_at(const ArenaDev& A, const SlotView& V, int 
 template <typename T>
 __device__ __forceinline__ T alpha_of(const ArenaDev& A, const SlotRec& S, int idx) {
   T a = sizeof(T) == 8 ? (T)S.alpha64 : (T)S.alpha;
-  if (S.hetero) a = (T)A.ALPHA[idx];
+  if (S.hetero) a = (T)ld(A.ALPHA, idx);
   if (S.needs_vf) {
-    const uint8_t f = A.VF[idx];
+    const uint8_t f = ld(A.VF, idx);
     if ((f & PHS_VF_SCAR) || ((S.cur_sched & PHS_SCHED_TEMP_BLOCK) && (f & PHS_VF_TEMP_BLOCK)))
       a = (T)0;
   }
@@ -260,9 +298,9 @@ __global__ void __launch_bounds__(kBlock) k_reaction(ArenaDev A) {
       const BrickRec& br = A.bricks[b];
       valid[j] = lane_bit(br.mask);
       idx[j] = br.start + (valid[j] ? mbcnt64(br.mask) : 0);
-      u[j] = A.U[idx[j]];
-      h[j] = A.H[idx[j]];
-      f[j] = A.VF[idx[j]];
+      u[j] = ld(A.U, idx[j]);
+      h[j] = ld(A.H, idx[j]);
+      f[j] = ld(A.VF, idx[j]);
     }
   }
 #pragma unroll
@@ -282,8 +320,8 @@ __global__ void __launch_bounds__(kBlock) k_reaction(ArenaDev A) {
     rhs32(S, u[j] + 0.5f * dt * k2u, h[j] + 0.5f * dt * k2h, J, k3u, k3h);
     rhs32(S, u[j] + dt * k3u, h[j] + dt * k3h, J, k4u, k4h);
     const float s6 = dt * (1.0f / 6.0f);
-    A.UO[idx[j]] = u[j] + s6 * (k1u + 2.0f * k2u + 2.0f * k3u + k4u);
-    A.HO[idx[j]] = h[j] + s6 * (k1h + 2.0f * k2h + 2.0f * k3h + k4h);
+    stv(A.UO, idx[j], u[j] + s6 * (k1u + 2.0f * k2u + 2.0f * k3u + k4u));
+    stv(A.HO, idx[j], h[j] + s6 * (k1h + 2.0f * k2h + 2.0f * k3h + k4h));
   }
 }
 
@@ -311,14 +349,14 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
     if (on[j]) {
       st[j].init(br);
       sv[j].load(A.slots[br.slot], 0);
-      un[j] = A.U[st[j].idx];
+      un[j] = ld(A.U, st[j].idx);
       st[j].gather(A.U, gu[j]);
       if (GUESS >= 1) {
-        d1[j] = A.D1[st[j].idx];
+        d1[j] = ld(A.D1, st[j].idx);
         st[j].gather((const float*)A.D1, g1[j]);
       }
       if (GUESS == 2) {
-        d2[j] = A.D2[st[j].idx];
+        d2[j] = ld(A.D2, st[j].idx);
         st[j].gather((const float*)A.D2, g2[j]);
       }
     }
@@ -339,7 +377,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
     }
     if (!st[j].valid || !sv[j].live) continue;
     const double a = alpha_at<double>(A, sv[j], st[j].idx);
-    A.R0[st[j].idx] = (float)(2.0 * a * s - (gn - a * sg));
+    stv(A.R0, st[j].idx, (float)(2.0 * a * s - (gn - a * sg)));
   }
 }
 
@@ -393,9 +431,9 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, const float
       st[j].init(br);
       sv[j].load(A.slots[br.slot], k);
       const int i = st[j].idx;
-      dn[j] = src[i];
-      r0[j] = A.R0[i];
-      dp[j] = prev_kind == 2 ? prev[i] : 0.0f;
+      dn[j] = ld(src, i);
+      r0[j] = ld(A.R0, i);
+      dp[j] = prev_kind == 2 ? ld(prev, i) : 0.0f;
       st[j].gather(src, g6[j]);
     }
   }
@@ -414,15 +452,15 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, const float
     const float dpv = prev_kind == 1 ? r0[j] * inv_theta : dp[j];
     const float dnew = sv[j].ca * d + sv[j].cb * dpv + sv[j].cg * (r0[j] - Ad);
     if (MODE == MODE_MID) {
-      out[i] = dnew;
+      stv(out, i, dnew);
     } else if (MODE == MODE_FINAL_X) {
-      const float us = A.U[i];
+      const float us = ld(A.U, i);
       float dt = dnew;
       if (GUESS >= 1) {
-        const float o1 = A.D1[i];
-        dt += guess_of(GUESS, o1, GUESS == 2 ? A.D2[i] : 0.0f);
-        if (GUESS == 2) A.D2[i] = o1;
-        A.D1[i] = dt;
+        const float o1 = ld(A.D1, i);
+        dt += guess_of(GUESS, o1, GUESS == 2 ? ld(A.D2, i) : 0.0f);
+        if (GUESS == 2) stv(A.D2, i, o1);
+        stv(A.D1, i, dt);
       }
       if (A.diag) {
         A.USTAR[i] = us;

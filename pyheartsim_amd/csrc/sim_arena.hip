@@ -341,8 +341,8 @@ int phs_arena_create(const phs_arena_config* cfg, phs_arena** handle) {
   if (!cfg || !handle) return phs::fail(PHS_ERR_ARG, "NULL argument");
   if (cfg->max_voxels <= 0 || cfg->max_bricks <= 0 || cfg->max_sims <= 0)
     return phs::fail(PHS_ERR_ARG, "capacities must be > 0");
-  if (cfg->max_voxels >= (1LL << 31) || cfg->max_bricks >= (1LL << 28))
-    return phs::fail(PHS_ERR_ARG, "capacity exceeds 32-bit compact indexing");
+  if (cfg->max_voxels >= (1LL << 29) || cfg->max_bricks >= (1LL << 28))
+    return phs::fail(PHS_ERR_ARG, "capacity exceeds 32-bit byte offsets (max 2^29 voxels)");
   if (!shape_supported(cfg->brick))
     return phs::fail(PHS_ERR_ARG, "brick shape must be one of 4x4x4, 8x8x1, 8x4x2, 4x8x2, 16x4x1");
   if (cfg->sweeps < 2 || cfg->sweeps > kMaxSweeps)
@@ -463,7 +463,15 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
     for (int k = 0; k < 6; ++k) {
       const int32_t nb = P.nbb[(size_t)b * 6 + k];
       r.nmask[k] = nb >= 0 ? P.mask[nb] : 0;
-      r.nstart[k] = nb >= 0 ? (int32_t)(vo + P.start[nb]) : 0;
+      int32_t below = 0;
+      if (nb >= 0 && (k & 1)) {
+        // - face: the gather rank is counted from the face-adjacent offset (B-1)*stride
+        const int bd[3] = {a->cfg.brick[0], a->cfg.brick[1], a->cfg.brick[2]};
+        const int stride[3] = {1, bd[0], bd[0] * bd[1]};
+        const int off = (bd[k >> 1] - 1) * stride[k >> 1];
+        below = off > 0 ? __builtin_popcountll(P.mask[nb] & ((1ULL << off) - 1ULL)) : 0;
+      }
+      r.nstart[k] = nb >= 0 ? (int32_t)(vo + P.start[nb] + below) : 0;
     }
     r.pad[0] = r.pad[1] = 0;
     hb[b] = r;
