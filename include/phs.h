@@ -94,6 +94,8 @@ typedef struct phs_outputs {
                               double for the plain FP64 form when sample_fp64 = 1) */
   const int32_t* probe_ids;/* host [n_probes] voxel ids */
   int32_t n_probes;
+  int32_t sample_ring;     /* bricked arena: > 0 = the sample buffers hold sample_ring rows used
+                              cyclically (sample s goes to row s % sample_ring) */
 } phs_outputs;
 
 int phs_version(void);

@@ -50,6 +50,7 @@ class Outputs(C.Structure):
         ("probe_u", C.c_void_p),
         ("probe_ids", C.c_void_p),
         ("n_probes", C.c_int32),
+        ("sample_ring", C.c_int32),
     ]
 
 

@@ -130,8 +130,7 @@ def shell_ms(delta: float = 0.4, t_final: float = 2000.0, seed: int = 0, n_elect
         node_flag[s1] = 0
     sim, arr, hm = assign_simulation_parameters(
         g, s1, None, node_flag, heart_model_flag=heart_model_flag, arrhythmia_flag=0,
-        t_final=t_final * (6 if heart_model_flag == 1 else 1),
-        dt=dt * (6 if heart_model_flag == 1 else 1),
+        t_final=t_final, dt=dt,   # physical ms; the AP time scaling divides both by 6
         pacing_start_time=10.0, pacing_cycle_length=cycle_length,
         ms=dict(tau_close=tau_close))
     if zero_c_in_scar:

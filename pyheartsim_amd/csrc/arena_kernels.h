@@ -52,7 +52,9 @@ struct SlotRec {
   int32_t max_beats;
   int32_t needs_vf;   // scar or temporary-block voxels present
   int32_t hetero;     // per-voxel alpha in the ALPHA array
-  float p[5];         // MS: 1/tau_open, 1/tau_close, 1/tau_in, 1/tau_out, v_gate; AP: k,a,eps0,mu1,mu2
+  int32_t ring;       // > 0: sample s is written to row s % ring
+  int32_t pad0;
+  float p[5];       // MS: 1/tau_open, 1/tau_close, 1/tau_in, 1/tau_out, v_gate; AP: k,a,eps0,mu1,mu2
   float dt;
   float alpha;
   float lat_thr;
@@ -397,7 +399,8 @@ __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotRec& S
     A.VF[idx] = (uint8_t)(f & ~PHS_VF_ABOVE);
   }
   if (S.cur_row >= 0) {
-    const int64_t o = S.cur_row * S.n_voxel + local;
+    const int64_t row = S.ring > 0 ? S.cur_row % S.ring : S.cur_row;
+    const int64_t o = row * S.n_voxel + local;
     if (S.su) S.su[o] = x;
     if (S.sh) S.sh[o] = A.H[idx];
   }

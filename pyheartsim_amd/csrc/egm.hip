@@ -192,41 +192,51 @@ __global__ void egm_reduce(const float* __restrict__ ws, int splits, int64_t T, 
   out[i] = s;
 }
 
-__global__ void egm_ptp(const float* __restrict__ egm, int64_t T, int E, float* __restrict__ ptp) {
-  const int e = blockIdx.x;
-  float lo = INFINITY, hi = -INFINITY;
-  for (int64_t t = threadIdx.x; t < T; t += blockDim.x) {
-    const float v = egm[t * E + e];
-    lo = fminf(lo, v);
-    hi = fmaxf(hi, v);
-  }
-  __shared__ float slo[256], shi[256];
-  slo[threadIdx.x] = lo;
-  shi[threadIdx.x] = hi;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      slo[threadIdx.x] = fminf(slo[threadIdx.x], slo[threadIdx.x + s]);
-      shi[threadIdx.x] = fmaxf(shi[threadIdx.x], shi[threadIdx.x + s]);
+constexpr int kNormThreads = 1024;
+constexpr int kNormMaxE = 4096;
+
+// One workgroup: peak-to-peak of every electrode over time, their mean, and the scaling of
+// the whole (T x E) electrogram (simulation/unipolar_electrogram_equation_parts.py:118-124).
+// No allocation and no host synchronisation, so it can follow the contraction on a stream.
+__global__ void __launch_bounds__(kNormThreads) egm_normalize_k(float* __restrict__ egm, int64_t T,
+                                                                int E, double typical) {
+  __shared__ float slo[kNormThreads], shi[kNormThreads];
+  __shared__ float ptp[kNormMaxE];
+  __shared__ double scale;
+  const int tid = threadIdx.x;
+  const int ec = E < kNormThreads ? E : kNormThreads;
+  const int G = kNormThreads / ec;
+  for (int e0 = 0; e0 < E; e0 += ec) {
+    const int el = tid % ec, g = tid / ec;
+    const int e = e0 + el;
+    float lo = INFINITY, hi = -INFINITY;
+    if (g < G && e < E)
+      for (int64_t t = g; t < T; t += G) {
+        const float v = egm[t * E + e];
+        lo = fminf(lo, v);
+        hi = fmaxf(hi, v);
+      }
+    slo[tid] = lo;
+    shi[tid] = hi;
+    __syncthreads();
+    if (tid < ec && e0 + tid < E) {
+      float l = slo[tid], h = shi[tid];
+      for (int q = 1; q < G; ++q) {
+        l = fminf(l, slo[tid + q * ec]);
+        h = fmaxf(h, shi[tid + q * ec]);
+      }
+      ptp[e0 + tid] = h - l;
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) ptp[e] = shi[0] - slo[0];
-}
-
-__global__ void egm_scale(float* __restrict__ egm, int64_t n, const float* __restrict__ ptp, int E,
-                          double typical) {
-  __shared__ double scale;
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     double m = 0.0;
     for (int e = 0; e < E; ++e) m += ptp[e];
-    m /= E;
-    scale = typical / m;
+    scale = typical / (m / E);
   }
   __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    egm[i] = (float)(egm[i] * scale);
+  const int64_t n = T * E;
+  for (int64_t i = tid; i < n; i += kNormThreads) egm[i] = (float)(egm[i] * scale);
 }
 
 }  // namespace
@@ -360,16 +370,9 @@ int phs_egm_destroy(phs_egm* g) {
 
 int phs_egm_normalize(float* egm, int64_t T, int32_t E, double typical, void* stream) {
   if (!egm || T <= 0 || E <= 0) return phs::fail(PHS_ERR_ARG, "invalid normalize arguments");
-  hipStream_t s = (hipStream_t)stream;
-  float* ptp = nullptr;
-  PHS_TRY_ALLOC(ptp, sizeof(float) * E);
-  egm_ptp<<<E, 256, 0, s>>>(egm, T, E, ptp);
-  egm_scale<<<256, 256, 0, s>>>(egm, T * E, ptp, E, typical);
-  hipError_t e = hipGetLastError();
-  hipError_t e2 = hipStreamSynchronize(s);
-  hipFree(ptp);
-  if (e != hipSuccess || e2 != hipSuccess)
-    return phs::fail(PHS_ERR_HIP, std::string("normalize: ") + hipGetErrorString(e ? e : e2));
+  if (E > kNormMaxE) return phs::fail(PHS_ERR_ARG, "normalize supports at most 4096 electrodes");
+  egm_normalize_k<<<1, kNormThreads, 0, (hipStream_t)stream>>>(egm, T, E, typical);
+  PHS_HIP_CHECK(hipGetLastError());
   return PHS_OK;
 }
 

@@ -543,6 +543,7 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
   sr.max_beats = out ? out->max_beats : 0;
   sr.needs_vf = needs_vf ? 1 : 0;
   sr.hetero = hetero ? 1 : 0;
+  sr.ring = out ? std::max(0, out->sample_ring) : 0;
   if (d->model == 0) {
     sr.p[0] = (float)(1.0 / d->params[0]);
     sr.p[1] = (float)(1.0 / d->params[1]);

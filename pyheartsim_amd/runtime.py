@@ -276,10 +276,11 @@ class Arena:
         self.slots: dict[int, SlotOutputs] = {}
 
     def add(self, prep: Prepared, samples: bool = True, sample_h: bool = False,
-            max_beats: int = 7, stream=None) -> int:
+            max_beats: int = 7, ring: int = 0, stream=None) -> int:
+        """Add a simulation.  ring > 0 keeps only `ring` sample rows, reused cyclically."""
         torch = self.torch
         N = prep.n_voxel
-        T = prep.n_saved()
+        T = ring if ring > 0 else prep.n_saved()
         su = torch.empty((T, N), dtype=torch.float32, device="cuda") if samples else None
         sh = torch.empty((T, N), dtype=torch.float32, device="cuda") if (samples and sample_h) else None
         lat = torch.empty((max_beats, N), dtype=torch.int32, device="cuda")
@@ -288,6 +289,7 @@ class Arena:
         out.samples_h = _lib.ptr(sh)
         out.lat = _lib.ptr(lat)
         out.max_beats = max_beats
+        out.sample_ring = ring
         desc = prep.desc()
         slot = C.c_int32(-1)
         _lib.check(self.lib.phs_arena_add(self.handle, C.byref(desc), C.byref(out), C.byref(slot),

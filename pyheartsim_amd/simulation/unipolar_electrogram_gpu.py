@@ -54,11 +54,14 @@ class LeadField:
                                            C.byref(self.handle), _lib.stream_ptr(stream)),
                    "phs_egm_create")
 
-    def run(self, samples, precision: str = "bf16x3", normalize: bool = True, stream=None):
-        """samples: CUDA float32 tensor (T x stride), first N columns used."""
+    def run(self, samples, precision: str = "bf16x3", normalize: bool = True, stream=None,
+            out=None):
+        """samples: CUDA float32 tensor (T x stride), first N columns used.  `out`: optional
+        contiguous (T x E) float32 destination."""
         import torch
         T, stride = samples.shape
-        out = torch.empty((T, self.E), dtype=torch.float32, device=samples.device)
+        if out is None:
+            out = torch.empty((T, self.E), dtype=torch.float32, device=samples.device)
         sp = _lib.stream_ptr(stream)
         _lib.check(self.lib.phs_egm_run(self.handle, _lib.ptr(samples), T, stride, _lib.ptr(out),
                                         PRECISION[precision], sp), "phs_egm_run")
