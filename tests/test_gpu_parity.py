@@ -102,3 +102,37 @@ def test_arena_alone_equals_mixed_and_graph(cuda):
     um, _, lm = mixed.outputs(sa)
     assert np.array_equal(ua.cpu().numpy(), um.cpu().numpy())
     assert np.array_equal(la.cpu().numpy(), lm.cpu().numpy())
+
+
+def test_fp32_arena_temp_block_and_scar_vs_oracle(cuda):
+    from pyheartsim_amd import runtime
+    w = small_shell(t_final=40.0, n_scar=2, temp_block=True)
+    ref = run_oracle(w, max_beats=7)
+    prep = _prep(w)
+    assert (prep.vflags & 1).any() and (prep.vflags & 2).any()
+    arena = runtime.Arena(prep.n_voxel, 1, runtime.SolverConfig())
+    slot = arena.add(prep)
+    arena.step(prep.n_steps)
+    _, _, lat = arena.outputs(slot)
+    assert _lat_ok(lat.cpu().numpy(), ref["lat"][:7], 1) <= 1
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_dropin_compute_matches_oracle(cuda, precision):
+    """simulation_gpu.compute keeps the reference signature and return layout."""
+    from pyheartsim_amd.simulation import simulation_gpu
+    w = small_shell(t_final=30.0, n_scar=1)
+    P = w["P_2d"]
+    ref = run_oracle(w, max_beats=7)
+    u, h, t, lat = simulation_gpu.compute(P.shape[0], P, w["geometry"], w["sim"], w["arr"],
+                                          precision=precision, return_lat=True)
+    assert u.dtype == np.float32 and h.dtype == np.float32 and t.dtype == np.float32
+    assert u.shape == ref["u"].shape and h.shape == ref["h"].shape
+    assert np.array_equal(t, ref["physical_time"])
+    if precision == "fp64":
+        # outputs are float32 (reference dtype): the bar is FP32 rounding of the FP64 state
+        assert np.max(np.abs(u - ref["u"])) <= 1e-6
+        assert np.array_equal(lat.astype(np.int64), ref["lat"][:7])
+    else:
+        assert _lat_ok(lat, ref["lat"][:7], 1) <= 1
+        assert np.max(np.abs(u - ref["u"])) <= 5e-2
