@@ -115,8 +115,119 @@ def cpu_baseline(work, u0, h0, step0, budget_s):
                 maxiter_hits=int(fails), seconds=dt)
 
 
+def run_ragged(args, torch, runtime):
+    """64 seeded shells of uneven size (0.4 mm, 1000 ms, 0-3 scar blobs, CL 250-600 ms) packed
+    into one arena; throughput over --steps graph-replayed steps; LAT of two of them against
+    the plain FP64 form over the full 1000 ms."""
+    from pyheartsim_amd import configs
+    t0 = time.perf_counter()
+    ws = configs.ragged(args.ragged_sims)
+    preps = []
+    for w in ws:
+        P = configs.build_P_2d(w["geometry"], w["sim"], w["hm"])
+        preps.append(runtime.prepare(P.shape[0], P, w["geometry"], w["sim"], w["arr"]))
+        del w["geometry"]["neighbor_id_2d"]
+    ns = [p.n_voxel for p in preps]
+    log(f"ragged: {len(preps)} sims, N {min(ns)}-{max(ns)}, total {sum(ns)}, setup "
+        f"{time.perf_counter()-t0:.1f}s")
+    solver = runtime.SolverConfig(sweeps=args.sweeps, guess=args.guess)
+    arena = runtime.Arena(sum(ns), len(preps), solver)
+    slots = [arena.add(p, samples=False) for p in preps]
+    G = args.graph_steps
+    ks = (args.steps // G) * G
+    t = timed(torch, lambda n: arena.step(n, graph=True), ks, (args.warmup // G) * G, G)
+    value = sum(ns) * ks / t
+    prof = arena.profile(20)
+    cnt = arena.counts()
+    log(f"ragged: {value:.4g} voxel-steps/s, {1e3*t/ks:.4f} ms/step, kernels {prof}, {cnt}")
+    # validation: run two of them to the end in the arena, compare with plain FP64
+    while min(arena.steps_done(s) for s in slots) < preps[0].n_steps:
+        arena.step(G, graph=True)
+    val = []
+    for i in (0, len(preps) // 2):
+        plain = runtime.PlainSim(preps[i], tol=1e-15, samples=False)
+        plain.run()
+        _, _, lat = arena.outputs(slots[i])
+        val.append(dict(sim=i, n_voxel=ns[i], lat_err_steps=lat_error(lat.cpu().numpy(),
+                                                                         plain.lat.cpu().numpy())))
+        del plain
+    log(f"ragged validation: {val}")
+    bytes_sweep = cnt["voxels"] * 16 + cnt["vf_voxels"] + cnt["bricks"] * 96
+    return dict(workload="ragged", sims=len(preps), voxels=sum(ns), voxel_steps_per_s=value,
+                ms_per_step=1e3 * t / ks, steps=ks, kernels_ms=prof,
+                sweep_gbs=bytes_sweep / (prof["sweep"] * 1e-3) / 1e9, validate=val)
+
+
+def run_precision(args, torch, runtime):
+    """The 0.2 mm shell with 3 scar blobs: FP32 sweeps with FP64-residual refinement, the
+    achieved CN residual, LAT against the plain FP64 form over a window, and the 256-electrode
+    x 2048-sample MFMA electrogram."""
+    from pyheartsim_amd import configs
+    from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
+    t0 = time.perf_counter()
+    w = configs.shell_fine(t_final=args.precision_ms)
+    g = w["geometry"]
+    P = configs.build_P_2d(g, w["sim"], w["hm"])
+    prep = runtime.prepare(P.shape[0], P, g, w["sim"], w["arr"])
+    N = prep.n_voxel
+    log(f"precision: N={N} steps={prep.n_steps} scar={int((prep.vflags & 1).sum())} setup "
+        f"{time.perf_counter()-t0:.1f}s")
+    out = dict(workload="precision", n_voxel=N, steps=prep.n_steps)
+    plain = runtime.PlainSim(prep, tol=1e-15, samples=False)
+    t0 = time.perf_counter()
+    plain.run()
+    torch.cuda.synchronize()
+    out["plain"] = dict(voxel_steps_per_s=N * prep.n_steps / (time.perf_counter() - t0),
+                        sweeps=plain.info()[0], lambda_max=plain.info()[2])
+    ref = plain.lat.cpu().numpy()
+    del plain
+    for refine in (0, 1, 2):
+        cfg = runtime.SolverConfig(sweeps=args.sweeps, guess=args.guess, refine=refine,
+                                   refine_sweeps=args.sweeps, diag=True)
+        arena = runtime.Arena(N, 1, cfg)
+        slot = arena.add(prep, samples=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        arena.step(prep.n_steps, graph=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res = arena.last_residual(slot)
+        _, _, lat = arena.outputs(slot)
+        out[f"refine{refine}"] = dict(voxel_steps_per_s=N * prep.n_steps / dt, cn_rel_residual=res,
+                                      lat_err_steps=lat_error(lat.cpu().numpy(), ref))
+        log(f"precision refine={refine}: {out[f'refine{refine}']}")
+        arena.close()
+    # electrogram: 256 electrodes x 2048 samples over all voxels, synthetic activation samples
+    E, T = 256, 2048
+    el = w["electrodes"]
+    lf = LeadField(el, g["voxel"], None, w["hm"]["c_voxel"], g["Delta"], g["neighbor_id_2d"])
+    x = torch.from_numpy(g["voxel"][:, 0].astype(np.float32)).cuda()
+    tt = torch.arange(T, device="cuda", dtype=torch.float32)[:, None]
+    samples = torch.sigmoid((tt * 0.03 - (x[None, :] + 25.0) * 0.3))   # a plane wave in x
+    e_ref = lf.run(samples, precision="fp32", normalize=False)
+    e_bf = lf.run(samples, precision="bf16x3", normalize=False)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(3):
+        lf.run(samples, precision="bf16x3", normalize=False, out=e_bf)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / 3
+    useful = 2.0 * E * T * N
+    out["egm"] = dict(electrodes=E, samples=T, ms=ms, useful_tflops=useful / ms / 1e9,
+                      mfma_issued_tflops=3 * useful / ms / 1e9,
+                      mfma_frac=3 * useful / ms / 1e9 / BF16_PEAK_TFLOPS,
+                      rel_l2_vs_fp32_mfma=float(((e_bf - e_ref).norm() / e_ref.norm()).item()))
+    log(f"precision egm: {out['egm']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--workload", choices=["shell_ms", "ragged", "precision"], default="shell_ms")
+    ap.add_argument("--ragged-sims", type=int, default=64)
+    ap.add_argument("--precision-ms", type=float, default=200.0)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=200)
@@ -141,6 +252,10 @@ def main():
     from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.workload != "shell_ms":
+        fn = run_ragged if args.workload == "ragged" else run_precision
+        print(json.dumps(fn(args, torch, runtime)), flush=True)
+        return
     t_setup = time.perf_counter()
     g, work, preps = build_workload(range(args.batch))
     prep = preps[0]

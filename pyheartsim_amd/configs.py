@@ -139,9 +139,18 @@ def shell_ms(delta: float = 0.4, t_final: float = 2000.0, seed: int = 0, n_elect
     return dict(name="shell_ms", geometry=g, sim=sim, arr=arr, hm=hm, electrodes=electrodes)
 
 
-def shell_fine(delta: float = 0.2, t_final: float = 2000.0, seed: int = 0):
-    return shell_ms(delta=delta, t_final=t_final, seed=seed, n_electrodes=256, n_scar=3,
-                    zero_c_in_scar=True)
+def shell_fine(delta: float = 0.2, t_final: float = 2000.0, seed: int = 0,
+               scar_fraction: float = 0.15):
+    """shell_ms physics at 0.2 mm with 3 seeded scar blobs (radii drawn in 3-6 mm, then
+    scaled together to cover `scar_fraction` of the wall), zero conductivity in the scar for
+    both the diffusion (node_flag 4) and the electrogram (c_voxel 0), 256 electrodes."""
+    w = shell_ms(delta=delta, t_final=t_final, seed=seed, n_electrodes=256)
+    g = w["geometry"]
+    nf = geo.add_scar_fraction(g, 3, (3.0, 6.0), scar_fraction, seed + 2)
+    nf[np.atleast_1d(w["arr"]["s1_pacing_voxel_id"])] = 0
+    w["arr"]["node_flag"] = nf
+    w["hm"]["c_voxel"] = np.where(nf == 4, 0.0, w["hm"]["c_voxel"])
+    return w
 
 
 def ragged(n_sims: int = 64, t_final: float = 1000.0, seed: int = 7, delta: float = 0.4,

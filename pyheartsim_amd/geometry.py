@@ -166,6 +166,37 @@ def add_scar(geometry: dict, n_blobs: int, radius_range=(3.0, 6.0), seed: int = 
     return flag
 
 
+def add_scar_fraction(geometry: dict, n_blobs: int, radius_range=(3.0, 6.0), fraction: float = 0.15,
+                      seed: int = 0, node_flag: np.ndarray | None = None) -> np.ndarray:
+    """node_flag with n_blobs seeded spherical scar blobs covering ~`fraction` of the voxels.
+
+    Centres and relative radii are seeded as in add_scar (radii drawn in radius_range); one
+    common scale of the radii is then found by bisection so that the scarred fraction of the
+    wall reaches `fraction`."""
+    v = geometry["voxel"]
+    n = v.shape[0]
+    rng = np.random.default_rng(seed)
+    centres = v[rng.integers(n, size=n_blobs)]
+    radii = rng.uniform(*radius_range, size=n_blobs)
+    d2 = np.stack([np.sum((v - c) ** 2, axis=1) for c in centres])
+
+    def covered(scale):
+        return np.any(d2 <= (radii[:, None] * scale) ** 2, axis=0)
+
+    lo, hi = 0.0, 1.0
+    while covered(hi).mean() < fraction and hi < 64:
+        hi *= 2
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        if covered(mid).mean() < fraction:
+            lo = mid
+        else:
+            hi = mid
+    flag = np.zeros(n, dtype=np.int64) if node_flag is None else node_flag.copy()
+    flag[covered(hi)] = 4
+    return flag
+
+
 def fibonacci_electrodes(n: int, radius: float = 15.0, centre=(0.0, 0.0, 0.0)) -> np.ndarray:
     """n points spread evenly on a sphere (Fibonacci lattice)."""
     k = np.arange(n, dtype=np.float64) + 0.5
