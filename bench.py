@@ -76,15 +76,25 @@ def timed(torch, fn, steps, warmup, chunk):
     return time.perf_counter() - t0
 
 
-def lat_error(a, b):
+def lat_error(a, b, n_steps=None):
+    """LAT difference (steps) of two (beats x voxels) tables, -1 = no activation.  With
+    n_steps, an activation in the last step of the window that the other form lacks counts
+    as a 1-step difference (the other form crosses one step later, outside the window)."""
     a = a.astype(np.int64)
     b = b.astype(np.int64)
     both = (a >= 0) & (b >= 0)
-    mismatch = int(np.count_nonzero((a >= 0) != (b >= 0)))
+    only = (a >= 0) != (b >= 0)
+    at_end = 0
+    if n_steps is not None:
+        last = only & (np.maximum(a, b) == n_steps - 1)
+        at_end = int(np.count_nonzero(last))
+        only &= ~last
+    mismatch = int(np.count_nonzero(only))
     d = np.abs(a[both] - b[both])
-    return dict(max=int(d.max()) if d.size else 0,
+    return dict(max=max(int(d.max()) if d.size else 0, 1 if at_end else 0),
                 p99=float(np.percentile(d, 99)) if d.size else 0.0,
-                activation_mismatch=mismatch, compared=int(d.size))
+                activation_mismatch=mismatch, activation_at_window_end=at_end,
+                compared=int(d.size))
 
 
 def cpu_baseline(work, u0, h0, step0, budget_s):
@@ -139,7 +149,8 @@ def run_ragged(args, torch, runtime):
     value = sum(ns) * ks / t
     prof = arena.profile(20)
     cnt = arena.counts()
-    log(f"ragged: {value:.4g} voxel-steps/s, {1e3*t/ks:.4f} ms/step, kernels {prof}, {cnt}")
+    log(f"ragged: {value:.4g} voxel-steps/s, {1e3*t/ks:.4f} ms/step, sweeps {arena.sweeps()}, "
+        f"kernels {prof}, {cnt}")
     # validation: run two of them to the end in the arena, compare with plain FP64
     while min(arena.steps_done(s) for s in slots) < preps[0].n_steps:
         arena.step(G, graph=True)
@@ -148,12 +159,14 @@ def run_ragged(args, torch, runtime):
         plain = runtime.PlainSim(preps[i], tol=1e-15, samples=False)
         plain.run()
         _, _, lat = arena.outputs(slots[i])
-        val.append(dict(sim=i, n_voxel=ns[i], lat_err_steps=lat_error(lat.cpu().numpy(),
-                                                                         plain.lat.cpu().numpy())))
+        val.append(dict(sim=i, n_voxel=ns[i],
+                        lat_err_steps=lat_error(lat.cpu().numpy(), plain.lat.cpu().numpy(),
+                                                preps[i].n_steps)))
         del plain
     log(f"ragged validation: {val}")
     bytes_sweep = cnt["voxels"] * 16 + cnt["vf_voxels"] + cnt["bricks"] * 96
     return dict(workload="ragged", sims=len(preps), voxels=sum(ns), voxel_steps_per_s=value,
+                sweeps=arena.sweeps(),
                 ms_per_step=1e3 * t / ks, steps=ks, kernels_ms=prof,
                 sweep_gbs=bytes_sweep / (prof["sweep"] * 1e-3) / 1e9, validate=val)
 
@@ -177,15 +190,19 @@ def run_precision(args, torch, runtime):
     t0 = time.perf_counter()
     plain.run()
     torch.cuda.synchronize()
-    out["plain"] = dict(voxel_steps_per_s=N * prep.n_steps / (time.perf_counter() - t0),
-                        sweeps=plain.info()[0], lambda_max=plain.info()[2])
+    plain_rate = N * prep.n_steps / (time.perf_counter() - t0)
+    out["plain"] = dict(voxel_steps_per_s=plain_rate, sweeps=plain.info()[0],
+                        lambda_max=plain.info()[2])
     ref = plain.lat.cpu().numpy()
     del plain
-    for refine in (0, 1, 2):
+    # automatic sweep count (stability + accuracy rule), without and with one FP64-residual
+    # refinement pass of 8 sweeps
+    for refine in (0, 1):
         cfg = runtime.SolverConfig(sweeps=args.sweeps, guess=args.guess, refine=refine,
-                                   refine_sweeps=args.sweeps, diag=True)
+                                   refine_sweeps=8, diag=True)
         arena = runtime.Arena(N, 1, cfg)
         slot = arena.add(prep, samples=False)
+        lmax = arena.bounds(slot)[1]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         arena.step(prep.n_steps, graph=False)
@@ -193,8 +210,11 @@ def run_precision(args, torch, runtime):
         dt = time.perf_counter() - t0
         res = arena.last_residual(slot)
         _, _, lat = arena.outputs(slot)
-        out[f"refine{refine}"] = dict(voxel_steps_per_s=N * prep.n_steps / dt, cn_rel_residual=res,
-                                      lat_err_steps=lat_error(lat.cpu().numpy(), ref))
+        rate = N * prep.n_steps / dt
+        out[f"refine{refine}"] = dict(
+            voxel_steps_per_s=rate, speedup_vs_plain=rate / plain_rate, sweeps=arena.sweeps(),
+            stable_sweeps=runtime.stable_sweeps(lmax, args.guess, refine, 8)[0],
+            lambda_max=lmax, cn_rel_residual=res, lat_err_steps=lat_error(lat.cpu().numpy(), ref, prep.n_steps))
         log(f"precision refine={refine}: {out[f'refine{refine}']}")
         arena.close()
     # electrogram: 256 electrodes x 2048 samples over all voxels, synthetic activation samples
@@ -227,12 +247,13 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--workload", choices=["shell_ms", "ragged", "precision"], default="shell_ms")
     ap.add_argument("--ragged-sims", type=int, default=64)
-    ap.add_argument("--precision-ms", type=float, default=200.0)
+    ap.add_argument("--precision-ms", type=float, default=300.0)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--sweeps", type=int, default=6)
+    ap.add_argument("--sweeps", type=int, default=0,
+                    help="Chebyshev sweeps per CN solve; 0 = automatic (stability rule)")
     ap.add_argument("--guess", type=int, default=1)
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--qps", type=float, default=0.5)
@@ -307,7 +328,7 @@ def main():
         su_c, _, lat_c = single.outputs(slot)
         lat_p = plain.lat.cpu().numpy()
         lat_cn = lat_c.cpu().numpy()
-        le = lat_error(lat_cn, lat_p)
+        le = lat_error(lat_cn, lat_p, S_full)
         T = prep.n_saved()
         vm = float((su_c[:T] - plain.su[:T]).abs().max().item())
         res["validate"] = dict(lat_err_steps=le, max_abs_dVm=vm, beats=int((lat_p[:, 0] >= 0).sum()))
@@ -348,6 +369,7 @@ def main():
     for p in preps:
         arena.add(p, samples=False)
     cnt = arena.counts()
+    sweeps_used = arena.sweeps()
     t = timed(torch, lambda n: arena.step(n, graph=True), ks, (W // G) * G, G)
     value = cnt["voxels"] * ks / t
     ms_step = 1e3 * t / ks
@@ -404,7 +426,7 @@ def main():
             "workload": f"shell_ms: MS ellipsoidal LA shell 25/20/18 mm, 3 mm wall, 4 vein holes, "
                         f"0.4 mm voxels (N={N}), dt 0.05 ms, CL 500 ms, x{B} simulations packed "
                         f"(different seeded pacing sites)",
-            "n_voxel": N, "batch": B, "steps_per_sim": S_full, "sweeps": args.sweeps,
+            "n_voxel": N, "batch": B, "steps_per_sim": S_full, "sweeps": sweeps_used,
             "guess": args.guess, "brick": [4, 4, 4], "graph_steps": G,
         },
         "speedup_vs_plain_fp64": value / res["plain"]["voxel_steps_per_s"],

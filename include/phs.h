@@ -102,6 +102,19 @@ int phs_version(void);
 const char* phs_last_error(void);
 int phs_device_count(int32_t* count);
 
+/* Host only.  Time-step stability of the inexact CN step of the bricked arena: with
+ * spec(I - dt/2 L) in [1, lambda_max], a Chebyshev solve of `sweeps` sweeps (plus `refine`
+ * FP64-residual passes of `refine_sweeps`) started from warm start `guess` (0, 1, 2) grows
+ * the stiffest modes by *amplification per step (> 1 = unstable; the exact solve gives the
+ * CN factor, <= 1).  phs_stable_sweeps gives the fewest sweeps with amplification <= 1
+ * (*stable) and the arena's automatic count (*automatic, may be NULL): the fewest stable
+ * sweeps whose error bound at the stiffest mode, lambda_max * max|P|, is <= 1/4.
+ * No counterpart in the reference, whose BiCGSTAB solve runs to a tolerance. */
+int phs_stable_sweeps(double lambda_max, int32_t guess, int32_t refine, int32_t refine_sweeps,
+                      int32_t* stable, int32_t* automatic);
+int phs_step_amplification(double lambda_max, int32_t guess, int32_t sweeps, int32_t refine,
+                           int32_t refine_sweeps, double* amplification);
+
 /* Host only (no device work): the Morton brick plan of a voxel set for brick shape
  * brick[3] (bx*by*bz = 64).  perm [n_voxel] receives the compact -> caller order,
  * *n_bricks the brick count, masks (NULL or capacity n_voxel) the per-brick occupancy.
@@ -144,7 +157,10 @@ typedef struct phs_arena_config {
   int64_t max_bricks;      /* brick capacity */
   int32_t max_sims;        /* slot capacity */
   int32_t brick[3];        /* brick shape (bx, by, bz), bx*by*bz = 64 */
-  int32_t sweeps;          /* Chebyshev sweeps of the first CN solve (>= 2) */
+  int32_t sweeps;          /* Chebyshev sweeps of the first CN solve: 2..64, or 0 = automatic
+                              (per slot the count of phs_stable_sweeps, the arena runs the
+                              largest over resident slots); a fixed count that is unstable
+                              for a slot's spectrum makes phs_arena_add fail */
   int32_t refine;          /* FP64-residual refinement passes (0..4) */
   int32_t refine_sweeps;   /* Chebyshev sweeps per refinement (>= 2) */
   int32_t xcd_remap;       /* 1 = map consecutive bricks to one XCD */
@@ -180,6 +196,8 @@ int phs_arena_slot_bounds(phs_arena* a, int32_t slot, double* lambda_min, double
 /* FP64 relative residual ||b - A x|| / ||b|| of the last CN solve of a slot, recomputed on
  * the device from the stored correction (diagnostic; refine > 0 keeps the correction). */
 int phs_arena_last_residual(phs_arena* a, int32_t slot, double* rel_residual, void* stream);
+/* Sweeps of the first CN solve that the next step runs (fixed, or automatic over slots). */
+int phs_arena_sweeps(phs_arena* a, int32_t* sweeps);
 /* Average duration (ms, HIP events on `stream`) of the reaction, right-hand-side and
  * Chebyshev-sweep kernels over `reps` launches each: ms[0..2].  Simulation state is not
  * changed (the reaction writes to scratch, the sweep runs in its intermediate mode). */

@@ -83,6 +83,8 @@ SIGNATURES = {
     "phs_version": (C.c_int, []),
     "phs_last_error": (C.c_char_p, []),
     "phs_device_count": (C.c_int, [_PI32]),
+    "phs_stable_sweeps": (C.c_int, [_D, _I32, _I32, _I32, _PI32, _PI32]),
+    "phs_step_amplification": (C.c_int, [_D, _I32, _I32, _I32, _I32, _PD]),
     "phs_plan_build": (C.c_int, [_I32, _P, _P, _I32, _P, _P, _PI32, _P]),
     "phs_plain_create":(C.c_int, [C.POINTER(SimDesc), C.POINTER(Outputs), _D, C.POINTER(_P)]),
     "phs_plain_run": (C.c_int, [_P, _I64, _P]),
@@ -99,6 +101,7 @@ SIGNATURES = {
     "phs_arena_state": (C.c_int, [_P, _I32, _P, _P, _P]),
     "phs_arena_slot_bounds": (C.c_int, [_P, _I32, _PD, _PD]),
     "phs_arena_last_residual": (C.c_int, [_P, _I32, _PD, _P]),
+    "phs_arena_sweeps": (C.c_int, [_P, _PI32]),
     "phs_arena_profile": (C.c_int, [_P, _I32, _PD, _P]),
     "phs_arena_counts": (C.c_int, [_P, _PI64, _PI64, _PI64, _PI64]),
     "phs_arena_destroy": (C.c_int, [_P]),

@@ -92,10 +92,18 @@ def select_sweeps(prep: runtime.Prepared, ref_lat, base: runtime.SolverConfig,
                   graph_steps: int = 50):
     """Smallest Chebyshev sweep count whose full-length run keeps every voxel's LAT within
     `max_lat_err` steps of `ref_lat` (the plain FP64 form's LAT, caller voxel order).
-    Returns (sweeps or None, table)."""
+    Counts below the stability limit of the simulation's spectral bound are skipped (the
+    arena refuses them).  Returns (sweeps or None, table)."""
     table = []
     chosen = None
+    probe = runtime.Arena(prep.n_voxel, 1, dataclasses.replace(base, sweeps=0))
+    lmax = probe.bounds(probe.add(prep, samples=False, max_beats=1))[1]
+    probe.close()
+    stable, _ = runtime.stable_sweeps(lmax, base.guess, base.refine, base.refine_sweeps)
     for m in sorted(candidates):
+        if m < stable:
+            table.append(dict(sweeps=m, guess=base.guess, lat_err_steps=None, unstable=True))
+            continue
         cfg = dataclasses.replace(base, sweeps=m)
         arena = runtime.Arena(prep.n_voxel, 1, cfg)
         slot = arena.add(prep, samples=False)

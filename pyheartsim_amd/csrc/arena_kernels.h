@@ -53,7 +53,7 @@ struct SlotRec {
   int32_t needs_vf;   // scar or temporary-block voxels present
   int32_t hetero;     // per-voxel alpha in the ALPHA array
   int32_t ring;       // > 0: sample s is written to row s % ring
-  int32_t pad0;
+  int32_t nsw;        // sweeps of this simulation's first CN solve (<= the launched count)
   float p[5];       // MS: 1/tau_open, 1/tau_close, 1/tau_in, 1/tau_out, v_gate; AP: k,a,eps0,mu1,mu2
   float dt;
   float alpha;
@@ -228,11 +228,21 @@ struct Stencil {
 };
 
 // Wave-uniform slot values a stencil kernel needs, loaded together.
+// Per-simulation sweep counts: a solve of K launched sweeps carries a simulation whose own
+// count is nsw < K by identity sweeps (x_{k+1} = x_k) from launch nsw on, so it computes
+// exactly what it computes alone.  Identity launches other than the last one only move the
+// value between the two ping-pong buffers, and only when the parity of the remaining
+// launches needs it: the simulation's waves skip every launch in [nsw, K-2] except launch
+// nsw when K - 1 - nsw is odd.  K = 0 (refinement solves, timing) disables all of this.
 struct SlotView {
   int live, needs_vf, hetero, cur_sched;
+  bool ident, skip;
   float alpha, inv_theta, ca, cb, cg;
   double alpha64;
-  __device__ __forceinline__ void load(const SlotRec& S, int k) {
+  __device__ __forceinline__ void load(const SlotRec& S, int k, int K = 0, bool last = true) {
+    const int nsw = S.nsw;
+    ident = K > 0 && k >= nsw;
+    skip = ident && !last && !(k == nsw && ((K - 1 - nsw) & 1));
     live = S.live;
     needs_vf = S.needs_vf;
     hetero = S.hetero;
@@ -411,8 +421,9 @@ __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotRec& S
 // GUESS: warm start of the correction (see guess_of); the final modes add it back and keep
 // the step's total correction in D1 (and the previous one in D2) for the next step.
 template <int BX, int BY, int BZ, int MODE, int NB, int GUESS>
-__global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, const float* cur, int cur_kind,
-                                                  const float* prev, int prev_kind, float* out) {
+__global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, const float* cur,
+                                                  int cur_kind, const float* prev, int prev_kind,
+                                                  float* out) {
   const int b0 = first_brick<NB>(A.xcd);
   const int lane = lane_id();
   const float* src = cur_kind ? A.R0 : cur;
@@ -431,8 +442,10 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, const float
       on[j] = br.slot >= 0;
     }
     if (on[j]) {
+      // the loads are issued before the skip test so that they depend on the brick record
+      // only; a skipped wave drops them
       st[j].init(br);
-      sv[j].load(A.slots[br.slot], k);
+      sv[j].load(A.slots[br.slot], k, K, MODE != MODE_MID);
       const int i = st[j].idx;
       dn[j] = ld(src, i);
       r0[j] = ld(A.R0, i);
@@ -442,7 +455,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, const float
   }
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
-    if (!on[j]) continue;
+    if (!on[j] || sv[j].skip) continue;
     const float inv_theta = sv[j].inv_theta;
     const float sc = cur_kind ? inv_theta : 1.0f;
     const float d = st[j].valid ? dn[j] * sc : 0.0f;
@@ -453,7 +466,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, const float
     const float a = alpha_at<float>(A, sv[j], i);
     const float Ad = d - a * s;
     const float dpv = prev_kind == 1 ? r0[j] * inv_theta : dp[j];
-    const float dnew = sv[j].ca * d + sv[j].cb * dpv + sv[j].cg * (r0[j] - Ad);
+    const float dnew = sv[j].ident ? d : sv[j].ca * d + sv[j].cb * dpv + sv[j].cg * (r0[j] - Ad);
     if (MODE == MODE_MID) {
       stv(out, i, dnew);
     } else if (MODE == MODE_FINAL_X) {

@@ -1,6 +1,9 @@
 // Error reporting, version, Chebyshev coefficients.
+#include <algorithm>
 #include <cmath>
+#include <complex>
 #include <string>
+#include <vector>
 
 #include "phs_internal.h"
 
@@ -29,6 +32,98 @@ void chebyshev_coefficients(double lmin, double lmax, int n, ChebCoef* out) {
   }
 }
 
+// Largest root modulus of a monic polynomial of degree n <= 3 (coefficients c[0..n-1] of
+// mu^{n-1} .. mu^0), by Durand-Kerner iteration.
+static double max_root_modulus(int n, const double* c) {
+  using cd = std::complex<double>;
+  cd z[3];
+  const cd seed(0.4, 0.9);
+  z[0] = 1.0;
+  for (int i = 1; i < n; ++i) z[i] = z[i - 1] * seed;
+  auto eval = [&](cd x) {
+    cd v = 1.0;
+    for (int i = 0; i < n; ++i) v = v * x + c[i];
+    return v;
+  };
+  for (int it = 0; it < 200; ++it) {
+    double change = 0.0;
+    for (int i = 0; i < n; ++i) {
+      cd den = 1.0;
+      for (int j = 0; j < n; ++j)
+        if (j != i) den *= z[i] - z[j];
+      if (std::abs(den) < 1e-300) den = 1e-300;
+      const cd step = eval(z[i]) / den;
+      z[i] -= step;
+      change = std::max(change, std::abs(step));
+    }
+    if (change < 1e-15) break;
+  }
+  double r = 0.0;
+  for (int i = 0; i < n; ++i) r = std::max(r, std::abs(z[i]));
+  return r;
+}
+
+double step_amplification(double lmax, int guess, int sweeps, int refine, int refine_sweeps) {
+  if (!(lmax > 1.0)) return 1.0;
+  const int kmax = std::max(sweeps, refine_sweeps);
+  std::vector<ChebCoef> cf(std::max(kmax, 1));
+  chebyshev_coefficients(1.0, lmax, kmax, cf.data());
+  // residual polynomial of m sweeps from the recurrence, evaluated at lam
+  auto poly = [&](double lam, int m) {
+    double ep = 1.0, e = 1.0 - cf[0].g * lam;
+    for (int k = 1; k < m; ++k) {
+      const double en = cf[k].a * e + cf[k].b * ep - cf[k].g * lam * e;
+      ep = e;
+      e = en;
+    }
+    return e;
+  };
+  const int n = 4096;
+  double worst = 0.0;
+  for (int i = 0; i <= n; ++i) {
+    const double lam = 1.0 + (lmax - 1.0) * i / n;
+    double P = poly(lam, sweeps);
+    for (int r = 0; r < refine; ++r) P *= poly(lam, refine_sweeps);
+    const double a = 1.0 + (1.0 - P) * 2.0 * (1.0 - lam) / lam;
+    double rad;
+    if (guess == 0) {
+      rad = std::fabs(a);
+    } else if (guess == 1) {
+      const double c[2] = {-(a + P), P};
+      rad = max_root_modulus(2, c);
+    } else {
+      const double c[3] = {-(a + 2.0 * P), 3.0 * P, -P};
+      rad = max_root_modulus(3, c);
+    }
+    worst = std::max(worst, rad);
+  }
+  return worst;
+}
+
+int stable_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax) {
+  for (int k = 2; k <= kmax; ++k)
+    if (step_amplification(lmax, guess, k, refine, refine_sweeps) <= 1.0 + 1e-9) return k;
+  return -1;
+}
+
+// Chebyshev bound on max |p_m| over [1, lmax]: 2 q^m / (1 + q^2m).
+static double cheb_bound(double lmax, int m) {
+  if (m <= 0) return 1.0;
+  const double s = std::sqrt(lmax);
+  const double q = (s - 1.0) / (s + 1.0);
+  const double qm = std::pow(q, m);
+  return 2.0 * qm / (1.0 + qm * qm);
+}
+
+int auto_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax) {
+  const int k0 = stable_sweeps(lmax, guess, refine, refine_sweeps, kmax);
+  if (k0 < 0) return -1;
+  const double tail = std::pow(cheb_bound(lmax, refine_sweeps), refine);
+  for (int k = k0; k <= kmax; ++k)
+    if (lmax * cheb_bound(lmax, k) * tail <= 0.25) return k;
+  return kmax;
+}
+
 int chebyshev_sweeps(double lmin, double lmax, double tol) {
   const double kappa = lmax / lmin;
   const double q = (std::sqrt(kappa) - 1.0) / (std::sqrt(kappa) + 1.0);
@@ -45,6 +140,29 @@ int chebyshev_sweeps(double lmin, double lmax, double tol) {
 extern "C" {
 
 int phs_version(void) { return 10000; }
+
+int phs_stable_sweeps(double lambda_max, int32_t guess, int32_t refine, int32_t refine_sweeps,
+                      int32_t* stable, int32_t* automatic) {
+  if (!stable) return phs::fail(PHS_ERR_ARG, "NULL argument");
+  if (guess < 0 || guess > 2 || refine < 0 || refine > 4 ||
+      (refine > 0 && (refine_sweeps < 2 || refine_sweeps > 64)) || !(lambda_max >= 1.0))
+    return phs::fail(PHS_ERR_ARG, "invalid stability query");
+  const int rs = refine > 0 ? refine_sweeps : 0;
+  *stable = phs::stable_sweeps(lambda_max, guess, refine, rs, 64);
+  if (automatic) *automatic = phs::auto_sweeps(lambda_max, guess, refine, rs, 64);
+  return *stable < 0 ? phs::fail(PHS_ERR_UNSUPPORTED, "no stable sweep count <= 64") : PHS_OK;
+}
+
+int phs_step_amplification(double lambda_max, int32_t guess, int32_t sweeps, int32_t refine,
+                           int32_t refine_sweeps, double* amplification) {
+  if (!amplification) return phs::fail(PHS_ERR_ARG, "NULL argument");
+  if (guess < 0 || guess > 2 || sweeps < 1 || sweeps > 64 || refine < 0 || refine > 4 ||
+      (refine > 0 && (refine_sweeps < 1 || refine_sweeps > 64)) || !(lambda_max >= 1.0))
+    return phs::fail(PHS_ERR_ARG, "invalid stability query");
+  *amplification = phs::step_amplification(lambda_max, guess, sweeps, refine,
+                                            refine > 0 ? refine_sweeps : 0);
+  return PHS_OK;
+}
 
 const char* phs_last_error(void) { return phs::g_last_error.c_str(); }
 

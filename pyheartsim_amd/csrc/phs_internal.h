@@ -24,6 +24,23 @@ void chebyshev_coefficients(double lmin, double lmax, int n, ChebCoef* out /* [n
 // Sweeps needed so that the Chebyshev error bound 2 q^m / (1 + q^2m) <= tol.
 int chebyshev_sweeps(double lmin, double lmax, double tol);
 
+// Time-step stability of the inexact CN step.  For an eigenmode of A with eigenvalue lam in
+// [1, lmax] (CN right-hand side b = 2(1 - lam) u / lam per unit mode), a solve whose error
+// polynomial is P(lam) = p_sweeps(lam) p_refine(lam)^refine maps
+//   guess 0: u' = a u                               a = 1 + (1 - P) 2 (1 - lam) / lam
+//   guess 1: (u, d) -> companion of mu^2 - (a + P) mu + P
+//   guess 2: (u, d, d_prev) -> companion of mu^3 - (a + 2P) mu^2 + 3P mu - P
+// (exact solve: P = 0, u' = (2/lam - 1) u, the CN factor).  Returns the largest root
+// modulus over a 4097-point grid of [1, lmax]; > 1 means the step grows that mode.
+double step_amplification(double lmax, int guess, int sweeps, int refine, int refine_sweeps);
+// Fewest sweeps (2..kmax) whose step amplification is <= 1, or -1.
+int stable_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax);
+// Automatic count: the fewest stable sweeps m for which the solve error at the stiffest
+// mode stays under a quarter of CN's own damping there, lmax * max|P| <= 1/4 with the
+// Chebyshev bound max|p_m| = 2 q^m / (1 + q^2m), q = (sqrt(lmax) - 1) / (sqrt(lmax) + 1)
+// (clamped to kmax; -1 when no count is stable).
+int auto_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax);
+
 }  // namespace phs
 
 #define PHS_HIP_CHECK(expr)                                                              \

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -84,6 +85,7 @@ struct SlotHost {
   int32_t* perm_dev = nullptr;
   uint8_t* sched_dev = nullptr;
   double lmin = 1.0, lmax = 1.0;
+  int32_t need_sweeps = 2;  // sweeps this slot asks of the arena (automatic mode)
   bool needs_vf = false;
   void* staging = nullptr;  // pinned upload buffer, freed after its copies complete
   hipEvent_t upload_done = nullptr;
@@ -115,6 +117,8 @@ struct phs_arena {
   hipGraph_t graph = nullptr;
   int32_t g_steps = -1;
   int64_t g_grid = -1;
+  int32_t g_sweeps = -1;
+  int32_t sweeps_eff = 2;  // sweeps of the first CN solve in the launched sequence
 };
 
 namespace {
@@ -160,23 +164,23 @@ int grid_for(const phs_arena* a) {
 // Launches of one time step for brick shape (BX, BY, BZ) and NB bricks per wavefront.
 template <int BX, int BY, int BZ, int NB, int GUESS>
 struct Step {
-  static void sweep(int mode, int grid, hipStream_t s, const ArenaDev& A, int k, const float* cur,
-                    int cur_kind, const float* prev, int prev_kind, float* out) {
+  static void sweep(int mode, int grid, hipStream_t s, const ArenaDev& A, int k, int K,
+                    const float* cur, int cur_kind, const float* prev, int prev_kind, float* out) {
     switch (mode) {
       case MODE_MID:
-        k_sweep<BX, BY, BZ, MODE_MID, NB, GUESS><<<grid, kBlock, 0, s>>>(A,k, cur, cur_kind, prev, prev_kind, out);
+        k_sweep<BX, BY, BZ, MODE_MID, NB, GUESS><<<grid, kBlock, 0, s>>>(A, k, K, cur, cur_kind, prev, prev_kind, out);
         break;
       case MODE_FINAL_X:
-        k_sweep<BX, BY, BZ, MODE_FINAL_X, NB, GUESS><<<grid, kBlock, 0, s>>>(A,k, cur, cur_kind, prev, prev_kind, out);
+        k_sweep<BX, BY, BZ, MODE_FINAL_X, NB, GUESS><<<grid, kBlock, 0, s>>>(A, k, K, cur, cur_kind, prev, prev_kind, out);
         break;
       case MODE_DACC_SET:
-        k_sweep<BX, BY, BZ, MODE_DACC_SET, NB, GUESS><<<grid, kBlock, 0, s>>>(A,k, cur, cur_kind, prev, prev_kind, out);
+        k_sweep<BX, BY, BZ, MODE_DACC_SET, NB, GUESS><<<grid, kBlock, 0, s>>>(A, k, K, cur, cur_kind, prev, prev_kind, out);
         break;
       case MODE_DACC_ADD:
-        k_sweep<BX, BY, BZ, MODE_DACC_ADD, NB, GUESS><<<grid, kBlock, 0, s>>>(A,k, cur, cur_kind, prev, prev_kind, out);
+        k_sweep<BX, BY, BZ, MODE_DACC_ADD, NB, GUESS><<<grid, kBlock, 0, s>>>(A, k, K, cur, cur_kind, prev, prev_kind, out);
         break;
       default:
-        k_sweep<BX, BY, BZ, MODE_DACC_ADD_X, NB, GUESS><<<grid, kBlock, 0, s>>>(A,k, cur, cur_kind, prev, prev_kind, out);
+        k_sweep<BX, BY, BZ, MODE_DACC_ADD_X, NB, GUESS><<<grid, kBlock, 0, s>>>(A, k, K, cur, cur_kind, prev, prev_kind, out);
     }
   }
 
@@ -188,24 +192,26 @@ struct Step {
     k_reaction<NB><<<grid, kBlock, 0, s>>>(A);
     k_rhs<BX, BY, BZ, NB, GUESS><<<grid, kBlock, 0, s>>>(A);
     float* bufs[2] = {a->DA, a->DB};
-    // One Chebyshev solve of m sweeps on the right-hand side held in R0.
-    auto solve = [&](int m, int last_mode) {
+    // One Chebyshev solve of m sweeps on the right-hand side held in R0; per_slot: each
+    // simulation runs its own count (<= m) of the first solve.
+    auto solve = [&](int m, int last_mode, bool per_slot) {
+      const int K = per_slot ? m : 0;
       for (int k = 1; k < m; ++k) {
         const float* cur = k == 1 ? nullptr : bufs[k % 2];
         const int cur_kind = k == 1 ? 1 : 0;
         const float* prev = k <= 2 ? nullptr : bufs[(k - 1) % 2];
         const int prev_kind = k == 1 ? 0 : (k == 2 ? 1 : 2);
-        sweep(k < m - 1 ? MODE_MID : last_mode, grid, s, A, k, cur, cur_kind, prev, prev_kind,
+        sweep(k < m - 1 ? MODE_MID : last_mode, grid, s, A, k, K, cur, cur_kind, prev, prev_kind,
               bufs[(k + 1) % 2]);
       }
     };
     if (c.refine <= 0) {
-      solve(c.sweeps, MODE_FINAL_X);
+      solve(a->sweeps_eff, MODE_FINAL_X, true);
     } else {
-      solve(c.sweeps, MODE_DACC_SET);
+      solve(a->sweeps_eff, MODE_DACC_SET, true);
       for (int r = 0; r < c.refine; ++r) {
         k_residual<BX, BY, BZ><<<grid1, kBlock, 0, s>>>(A);
-        solve(c.refine_sweeps, r == c.refine - 1 ? MODE_DACC_ADD_X : MODE_DACC_ADD);
+        solve(c.refine_sweeps, r == c.refine - 1 ? MODE_DACC_ADD_X : MODE_DACC_ADD, false);
       }
     }
     k_advance<<<(c.max_sims + 63) / 64, 64, 0, s>>>(a->slots, c.max_sims);
@@ -229,7 +235,7 @@ struct Step {
     for (int i = 0; i < reps; ++i) k_rhs<BX, BY, BZ, NB, GUESS><<<grid, kBlock, 0, s>>>(A);
     hipEventRecord(ev[2], s);
     for (int i = 0; i < reps; ++i)
-      k_sweep<BX, BY, BZ, MODE_MID, NB, GUESS><<<grid, kBlock, 0, s>>>(A,3, a->DA, 0, a->DB, 2, a->DB);
+      k_sweep<BX, BY, BZ, MODE_MID, NB, GUESS><<<grid, kBlock, 0, s>>>(A, 3, 0, a->DA, 0, a->DB, 2, a->DB);
     hipEventRecord(ev[3], s);
     hipError_t r = hipEventSynchronize(ev[3]);
     for (int i = 0; i < 3; ++i) {
@@ -328,9 +334,14 @@ void reap(phs_arena* a, bool wait) {
 
 void recompute_high(phs_arena* a) {
   int64_t hi = 0;
+  int32_t sw = 2;
   for (auto& h : a->sh)
-    if (h.used) hi = std::max(hi, h.brick_off + h.n_bricks);
+    if (h.used) {
+      hi = std::max(hi, h.brick_off + h.n_bricks);
+      sw = std::max(sw, h.need_sweeps);
+    }
   a->brick_high = hi;
+  a->sweeps_eff = a->cfg.sweeps > 0 ? a->cfg.sweeps : sw;
 }
 
 }  // namespace
@@ -345,8 +356,8 @@ int phs_arena_create(const phs_arena_config* cfg, phs_arena** handle) {
     return phs::fail(PHS_ERR_ARG, "capacity exceeds 32-bit byte offsets (max 2^29 voxels)");
   if (!shape_supported(cfg->brick))
     return phs::fail(PHS_ERR_ARG, "brick shape must be one of 4x4x4, 8x8x1, 8x4x2, 4x8x2, 16x4x1");
-  if (cfg->sweeps < 2 || cfg->sweeps > kMaxSweeps)
-    return phs::fail(PHS_ERR_ARG, "sweeps must be in [2, 64]");
+  if (cfg->sweeps != 0 && (cfg->sweeps < 2 || cfg->sweeps > kMaxSweeps))
+    return phs::fail(PHS_ERR_ARG, "sweeps must be 0 (automatic) or in [2, 64]");
   if (cfg->refine < 0 || cfg->refine > 4) return phs::fail(PHS_ERR_ARG, "refine must be in [0, 4]");
   if (cfg->bricks_per_wave != 0 && cfg->bricks_per_wave != 1 && cfg->bricks_per_wave != 2)
     return phs::fail(PHS_ERR_ARG, "bricks_per_wave must be 1 or 2");
@@ -355,6 +366,7 @@ int phs_arena_create(const phs_arena_config* cfg, phs_arena** handle) {
     return phs::fail(PHS_ERR_ARG, "refine_sweeps must be in [2, 64]");
   phs_arena* a = new phs_arena();
   a->cfg = *cfg;
+  a->sweeps_eff = cfg->sweeps > 0 ? cfg->sweeps : 2;
   const size_t nv = (size_t)cfg->max_voxels;
   hipError_t e = hipSuccess;
   const bool need_dacc = cfg->refine > 0 || cfg->diag;
@@ -513,6 +525,26 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
   h.lmin = 1.0;
   h.lmax = 1.0 + 2.0 * amax_deg;
   h.needs_vf = needs_vf;
+  {
+    // Inexact CN steps amplify the stiffest modes unless the sweep polynomial is small
+    // enough there; reject a fixed count that would, or pick one (automatic mode).
+    const phs_arena_config& c = a->cfg;
+    const int rs = c.refine > 0 ? c.refine_sweeps : 0;
+    const int need = phs::stable_sweeps(h.lmax, c.guess, c.refine, rs, kMaxSweeps);
+    char msg[256];
+    if (need < 0) {
+      std::snprintf(msg, sizeof msg, "no sweep count <= %d keeps the CN step stable for "
+                    "lambda_max = %.3f (guess %d)", kMaxSweeps, h.lmax, c.guess);
+      return undo(PHS_ERR_UNSUPPORTED, msg);
+    }
+    if (c.sweeps > 0 && c.sweeps < need) {
+      std::snprintf(msg, sizeof msg, "sweeps = %d makes the CN step unstable for lambda_max = "
+                    "%.3f (guess %d, refine %d): needs >= %d, or sweeps = 0 (automatic)",
+                    c.sweeps, h.lmax, c.guess, c.refine, need);
+      return undo(PHS_ERR_ARG, msg);
+    }
+    h.need_sweeps = c.sweeps > 0 ? c.sweeps : phs::auto_sweeps(h.lmax, c.guess, c.refine, rs, kMaxSweeps);
+  }
   std::vector<phs::ChebCoef> coef(kMaxSweeps);
   SlotRec sr;
   std::memset(&sr, 0, sizeof(sr));
@@ -544,6 +576,7 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
   sr.needs_vf = needs_vf ? 1 : 0;
   sr.hetero = hetero ? 1 : 0;
   sr.ring = out ? std::max(0, out->sample_ring) : 0;
+  sr.nsw = h.need_sweeps;
   if (d->model == 0) {
     sr.p[0] = (float)(1.0 / d->params[0]);
     sr.p[1] = (float)(1.0 / d->params[1]);
@@ -629,7 +662,7 @@ int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream) {
   if (a->brick_high == 0) return PHS_OK;
   hipStream_t s = (hipStream_t)stream;
   const int64_t grid = grid_for(a);
-  if (!a->gexec || a->g_steps != n_steps || a->g_grid != grid) {
+  if (!a->gexec || a->g_steps != n_steps || a->g_grid != grid || a->g_sweeps != a->sweeps_eff) {
     if (a->gexec) hipGraphExecDestroy(a->gexec);
     if (a->graph) hipGraphDestroy(a->graph);
     a->gexec = nullptr;
@@ -643,6 +676,7 @@ int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream) {
     PHS_HIP_CHECK(hipGraphInstantiate(&a->gexec, g, nullptr, nullptr, 0));
     a->g_steps = n_steps;
     a->g_grid = grid;
+    a->g_sweeps = a->sweeps_eff;
   }
   PHS_HIP_CHECK(hipGraphLaunch(a->gexec, s));
   for (auto& h : a->sh)
@@ -701,7 +735,16 @@ int phs_arena_last_residual(phs_arena* a, int32_t slot, double* rel, void* strea
   double acc[2];
   PHS_HIP_CHECK(hipMemcpyAsync(acc, a->diag_acc, 16, hipMemcpyDeviceToHost, s));
   PHS_HIP_CHECK(hipStreamSynchronize(s));
-  *rel = acc[1] > 0 ? std::sqrt(acc[0] / acc[1]) : 0.0;
+  if (acc[1] > 0)
+    *rel = std::sqrt(acc[0] / acc[1]);
+  else  // b = 0 with x = 0 is an exact solve; anything else (NaN included) is not
+    *rel = (acc[0] == 0.0 && acc[1] == 0.0) ? 0.0 : NAN;
+  return PHS_OK;
+}
+
+int phs_arena_sweeps(phs_arena* a, int32_t* sweeps) {
+  if (!a || !sweeps) return phs::fail(PHS_ERR_ARG, "NULL argument");
+  *sweeps = a->sweeps_eff;
   return PHS_OK;
 }
 

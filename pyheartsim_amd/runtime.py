@@ -227,9 +227,31 @@ class PlainSim:
 
 
 # --------------------------------------------------------------------- bricked FP32 arena
+def stable_sweeps(lambda_max: float, guess: int = 1, refine: int = 0, refine_sweeps: int = 8):
+    """(fewest Chebyshev sweeps whose inexact CN step grows no mode, the arena's automatic
+    count) for spectral bound lambda_max (phs_stable_sweeps)."""
+    lib = _lib.load()
+    s, a = C.c_int32(), C.c_int32()
+    _lib.check(lib.phs_stable_sweeps(float(lambda_max), guess, refine, refine_sweeps, C.byref(s),
+                                     C.byref(a)), "phs_stable_sweeps")
+    return s.value, a.value
+
+
+def step_amplification(lambda_max: float, sweeps: int, guess: int = 1, refine: int = 0,
+                       refine_sweeps: int = 8) -> float:
+    """Largest per-step growth factor of any mode of the inexact CN step (phs_step_amplification)."""
+    lib = _lib.load()
+    r = C.c_double()
+    _lib.check(lib.phs_step_amplification(float(lambda_max), guess, sweeps, refine, refine_sweeps,
+                                          C.byref(r)), "phs_step_amplification")
+    return r.value
+
+
 @dataclass
 class SolverConfig:
-    sweeps: int = 6
+    # 0 = automatic: per simulation the fewest sweeps that keep the step stable for its
+    # spectral bound, plus one; the arena runs the largest over resident simulations
+    sweeps: int = 0
     refine: int = 0
     refine_sweeps: int = 8
     brick: tuple = (4, 4, 4)
@@ -329,6 +351,12 @@ class Arena:
         _lib.check(self.lib.phs_arena_last_residual(self.handle, slot, C.byref(r), _lib.stream_ptr()),
                    "phs_arena_last_residual")
         return r.value
+
+    def sweeps(self) -> int:
+        """Sweeps of the first CN solve that the next step runs."""
+        s = C.c_int32()
+        _lib.check(self.lib.phs_arena_sweeps(self.handle, C.byref(s)), "phs_arena_sweeps")
+        return s.value
 
     def profile(self, reps: int = 50, stream=None):
         """Average kernel durations (ms): reaction, rhs, sweep (HIP events)."""

@@ -13,9 +13,12 @@ extras select the device form:
                        FP32, CN right-hand side in FP64)
     precision="fp64"   plain FP64 form (reference algorithm, tight CN solve)
     sweeps, guess, refine, refine_sweeps, brick   solver settings of the FP32 form
-                       (defaults: 6 Chebyshev sweeps warm-started from the previous
-                       step's CN correction, validated at LAT error <= 1 dt against the
-                       plain FP64 form on the 0.4 mm shell over 40 000 steps)
+                       (defaults: sweeps=0, the automatic count of Chebyshev sweeps
+                       warm-started from the previous step's CN correction: the fewest
+                       that keep the step stable for the geometry's spectral bound with
+                       the solve error at the stiffest mode under a quarter of CN's
+                       damping -- 6 on the 0.4 mm shell, 16 at 0.2 mm, both validated at
+                       LAT error <= 1 dt against the plain FP64 form)
     return_lat=True    also return the per-voxel activation-time table
                        (max_beats x N step indices, -1 = none)
     device_out=True    return torch device tensors instead of NumPy arrays
@@ -30,7 +33,7 @@ from .. import runtime
 
 
 def compute(n_voxel, P_2d, geometry_data, simulation_parameters, arrhythmia_parameters, *,
-            precision: str = "fp32", sweeps: int = 6, guess: int = 1, refine: int = 0,
+            precision: str = "fp32", sweeps: int = 0, guess: int = 1, refine: int = 0,
             refine_sweeps: int = 8, brick=(4, 4, 4), plain_tol: float = 1e-15,
             return_lat: bool = False, device_out: bool = False):
     prep = runtime.prepare(n_voxel, P_2d, geometry_data, simulation_parameters,

@@ -136,3 +136,47 @@ def test_dropin_compute_matches_oracle(cuda, precision):
     else:
         assert _lat_ok(lat, ref["lat"][:7], 1) <= 1
         assert np.max(np.abs(u - ref["u"])) <= 5e-2
+
+
+def test_per_simulation_sweep_counts_mixed_spacing(cuda):
+    """Automatic sweeps: a 0.25 mm shell (lambda_max ~ 20) needs more sweeps than a 0.5 mm
+    one (~ 6).  Packed together the arena launches the larger count, and each simulation
+    still computes bitwise what it computes alone (identity sweeps past its own count);
+    the fine one stays within 1 step of the FP64 oracle's LAT."""
+    from pyheartsim_amd import runtime
+    coarse = small_shell(t_final=20.0, seed=3)
+    fine = small_shell(delta=0.25, semi_axes=(6.0, 5.5, 5.0), wall=1.5, t_final=20.0, seed=4)
+    pc, pf = _prep(coarse), _prep(fine)
+    solver = runtime.SolverConfig()           # sweeps = 0: automatic
+    res = {}
+    for name, p in (("coarse", pc), ("fine", pf)):
+        a = runtime.Arena(p.n_voxel, 1, solver)
+        s = a.add(p)
+        lmax = a.bounds(s)[1]
+        assert a.sweeps() == runtime.stable_sweeps(lmax, solver.guess)[1]
+        a.step(p.n_steps)
+        res[name] = (a.sweeps(), [t.cpu().numpy() for t in a.outputs(s)[::2]])
+        a.close()
+    assert res["fine"][0] > res["coarse"][0]
+    mix = runtime.Arena(pc.n_voxel + pf.n_voxel, 2, solver)
+    sf = mix.add(pf)
+    sc = mix.add(pc)
+    assert mix.sweeps() == res["fine"][0]
+    for _ in range(max(pc.n_steps, pf.n_steps) // 50 + 1):
+        mix.step(50, graph=True)
+    for name, s in (("coarse", sc), ("fine", sf)):
+        um, _, lm = mix.outputs(s)
+        ua, la = res[name][1]
+        assert np.array_equal(ua, um.cpu().numpy()), name
+        assert np.array_equal(la, lm.cpu().numpy()), name
+    ref = run_oracle(fine, max_beats=7)
+    assert _lat_ok(res["fine"][1][1], ref["lat"][:7], 1) <= 1
+
+
+def test_fixed_sweeps_below_stability_limit_rejected(cuda):
+    from pyheartsim_amd import _lib, runtime
+    fine = small_shell(delta=0.25, semi_axes=(6.0, 5.5, 5.0), wall=1.5, t_final=5.0, seed=4)
+    p = _prep(fine)
+    a = runtime.Arena(p.n_voxel, 1, runtime.SolverConfig(sweeps=4))
+    with pytest.raises(_lib.PhsError, match="unstable"):
+        a.add(p)

@@ -1,0 +1,124 @@
+"""Time-step stability of the inexact CN step (host-only C ABI: phs_step_amplification,
+phs_stable_sweeps).
+
+Pins:
+  * the per-mode growth factor equals the spectral radius of the mode map built here
+    directly from its definition (u' = u + d, d = d* + P (d0 - d*), d* = b / lam) with
+    numpy.linalg.eigvals -- not from the characteristic polynomials the library uses;
+  * an exact solve (many sweeps) reduces to the CN factor |2/lam - 1| <= 1;
+  * a 1D chain with zero-flux ends, stepped on the CPU with the same Chebyshev recurrence,
+    grows without bound two sweep counts below the stability limit and stays bounded at
+    the limit (a plain-matrix restatement, no library code);
+  * the counts measured on the 0.2 mm shell (lambda_max = 31): 14 sweeps is the stability
+    boundary with warm start, 16 the automatic count; 6 for the 0.4 mm shell (8.5).
+"""
+import numpy as np
+import pytest
+
+from pyheartsim_amd import runtime
+
+
+def cheb_coefs(lmin, lmax, n):
+    th, de = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+    rp = de / th
+    out = [(0.0, 0.0, 1.0 / th)]
+    for _ in range(1, n):
+        r = 1.0 / (2.0 * th / de - rp)
+        out.append((1.0 + r * rp, -r * rp, 2.0 * r / de))
+        rp = r
+    return out
+
+
+def cheb_solve(apply_a, b, x0, lmax, m):
+    """m sweeps of the Chebyshev semi-iteration on A x = b from x0, spec(A) in [1, lmax]."""
+    c = cheb_coefs(1.0, lmax, m)
+    xp = x0
+    x = x0 + c[0][2] * (b - apply_a(x0))
+    for k in range(1, m):
+        a, bb, g = c[k]
+        x, xp = a * x + bb * xp + g * (b - apply_a(x)), x
+    return x
+
+
+def err_poly(lam, lmax, m):
+    return cheb_solve(lambda v: lam * v, 0.0 * lam, np.ones_like(lam), lmax, m)
+
+
+def mode_radius(lam, lmax, m, guess, refine=0, rs=8):
+    P = err_poly(np.array([lam]), lmax, m)[0]
+    for _ in range(refine):
+        P *= err_poly(np.array([lam]), lmax, rs)[0]
+    c = 2.0 * (1.0 - lam) / lam          # d* = c u
+    if guess == 0:
+        M = np.array([[1.0 + (1.0 - P) * c]])
+    elif guess == 1:                      # state (u, d_prev), d0 = d_prev
+        d = np.array([(1.0 - P) * c, P])
+        M = np.vstack([d, d])
+        M[0, 0] += 1.0
+    else:                                 # state (u, d_prev, d_prev2), d0 = 2 d_prev - d_prev2
+        d = np.array([(1.0 - P) * c, 2.0 * P, -P])
+        M = np.zeros((3, 3))
+        M[0] = d
+        M[0, 0] += 1.0
+        M[1] = d
+        M[2, 1] = 1.0
+    return np.max(np.abs(np.linalg.eigvals(M)))
+
+
+@pytest.mark.parametrize("lmax,m,guess,refine", [
+    (8.5, 6, 1, 0), (8.5, 4, 1, 0), (31.0, 6, 1, 0), (31.0, 12, 1, 0), (31.0, 16, 0, 0),
+    (31.0, 9, 2, 0), (31.0, 6, 1, 1), (61.0, 20, 1, 0), (16.0, 7, 0, 0)])
+def test_amplification_matches_mode_matrix(lmax, m, guess, refine):
+    lams = np.linspace(1.0, lmax, 4097)
+    want = max(mode_radius(l, lmax, m, guess, refine) for l in lams)
+    got = runtime.step_amplification(lmax, m, guess, refine, 8)
+    assert abs(got - want) <= 1e-9 * max(1.0, want)
+
+
+def test_exact_solve_reduces_to_cn_factor():
+    # 64 sweeps at lambda_max 8.5: |P| < 1e-30, the map is the CN factor, max 1 at lam = 1
+    assert abs(runtime.step_amplification(8.5, 64, 1) - 1.0) <= 1e-12
+    lam = np.linspace(1.0, 8.5, 101)
+    assert np.all(np.abs(2.0 / lam - 1.0) <= 1.0)
+
+
+def test_counts_measured_on_shells():
+    assert runtime.stable_sweeps(31.0, 1) == (14, 16)
+    assert runtime.stable_sweeps(8.5, 1) == (5, 6)
+    assert runtime.step_amplification(31.0, 13, 1) > 1.005
+    assert runtime.step_amplification(31.0, 14, 1) <= 1.0 + 1e-9
+
+
+def _chain_run(m, lmax_bound, n=48, steps=400, alpha=2.5):
+    # 1D chain, zero-flux mirror ends: (L u)_i = u_{i-1} + u_{i+1} - 2 u_i with a missing
+    # neighbour replaced by the one opposite (reference Neumann form); A = I - alpha L
+    L = np.zeros((n, n))
+    for i in range(n):
+        lo, hi = i - 1, i + 1
+        if lo < 0:
+            lo = hi
+        if hi >= n:
+            hi = lo
+        L[i, lo] += 1.0
+        L[i, hi] += 1.0
+        L[i, i] -= 2.0
+    A = np.eye(n) - alpha * L
+    rng = np.random.default_rng(0)
+    u = rng.standard_normal(n)
+    d_prev = np.zeros(n)
+    peak = 0.0
+    for _ in range(steps):
+        b = 2.0 * alpha * (L @ u)
+        d = cheb_solve(lambda v: A @ v, b, d_prev, lmax_bound, m)
+        u = u + d
+        d_prev = d
+        peak = max(peak, float(np.max(np.abs(u))))
+    return peak
+
+
+def test_chain_diverges_below_limit_and_not_at_it():
+    lmax = 1.0 + 2.0 * 2.5 * 2          # Gershgorin bound, alpha 2.5, one axis: 11
+    stable, _ = runtime.stable_sweeps(lmax, 1)
+    start = _chain_run(64, lmax, steps=1)
+    assert _chain_run(stable, lmax) <= 10.0 * start
+    assert _chain_run(stable - 2, lmax) > 1e3 * start
