@@ -106,17 +106,19 @@ def cpu_baseline(work, u0, h0, step0, budget_s):
     cs.u[:] = u0
     cs.h[:] = h0
     cs.step = step0
+    cs.run(2, threads)                 # warm-up (first-touch of the CSR arrays)
+    # chunks of `chunk` steps until the budget is spent; each call rebuilds the CSR operator
+    # (the reference rebuilds A every step, legacy/compute_simulation_cpu.py:131-134)
+    chunk = 50
+    steps = 0
+    fails = 0
     t0 = time.perf_counter()
-    cs.run(2, threads)
-    t1 = time.perf_counter()
-    cs.run(10, threads)
-    t2 = time.perf_counter()
-    # the operator build is per call; the per-step cost is the difference of the two calls
-    per = max((t2 - t1) - (t1 - t0), 1e-5) / 8
-    steps = max(10, min(20000, int(budget_s / per)))
-    t0 = time.perf_counter()
-    fails = cs.run(steps, threads)
-    dt = time.perf_counter() - t0
+    while True:
+        fails += cs.run(chunk, threads)
+        steps += chunk
+        dt = time.perf_counter() - t0
+        if dt >= budget_s or steps >= 20000:
+            break
     n = cs.n
     return dict(value=n * steps / dt, unit="voxel-steps/s", cores=threads, kind="port",
                 sample=f"{steps} steps from step {step0} (mid-activation) of the {n}-voxel shell: "

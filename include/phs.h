@@ -235,7 +235,7 @@ int phs_egm_create(int32_t n_voxel, int32_t n_electrode, const double* voxel,
                    phs_egm** handle, void* stream);
 /* samples: device float [n_time][sample_stride] (first n_voxel columns used).
  * egm: device float [n_time][n_electrode] (reference layout), un-normalised.
- * precision: 0 = BF16 hi/lo split MFMA, 1 = FP32 (vector ALU), 2 = single BF16 MFMA. */
+ * precision: 0 = BF16 hi/lo split MFMA, 1 = FP32 MFMA (16x16x4), 2 = single BF16 MFMA. */
 int phs_egm_run(phs_egm* handle, const float* samples, int64_t n_time, int64_t sample_stride,
                 float* egm, int32_t precision, void* stream);
 int phs_egm_destroy(phs_egm* handle);

@@ -90,18 +90,15 @@ def test_counts_measured_on_shells():
 
 
 def _chain_run(m, lmax_bound, n=48, steps=400, alpha=2.5):
-    # 1D chain, zero-flux mirror ends: (L u)_i = u_{i-1} + u_{i+1} - 2 u_i with a missing
-    # neighbour replaced by the one opposite (reference Neumann form); A = I - alpha L
+    # 1D chain with zero-flux ends: (L u)_i = sum over present neighbours j of (u_j - u_i),
+    # a missing neighbour contributing nothing (the reference's operator,
+    # legacy/compute_simulation_cpu.py:21-111); A = I - alpha L
     L = np.zeros((n, n))
     for i in range(n):
-        lo, hi = i - 1, i + 1
-        if lo < 0:
-            lo = hi
-        if hi >= n:
-            hi = lo
-        L[i, lo] += 1.0
-        L[i, hi] += 1.0
-        L[i, i] -= 2.0
+        for j in (i - 1, i + 1):
+            if 0 <= j < n:
+                L[i, j] += 1.0
+                L[i, i] -= 1.0
     A = np.eye(n) - alpha * L
     rng = np.random.default_rng(0)
     u = rng.standard_normal(n)
