@@ -15,7 +15,10 @@
 //     egm (T x E) = U (T x N) . Wt^T (N x E)
 // over the 1 kHz samples U.  It runs on MFMA (v_mfma_f32_16x16x32_bf16) as BF16 hi/lo split
 // products (hi*hi + hi*lo + lo*hi) with FP32 accumulation, split over K = N voxels with a
-// fixed-order reduction (deterministic, unlike the reference's atomics).
+// fixed-order reduction (deterministic, unlike the reference's atomics).  Samples and lead
+// field are staged through LDS per 64-deep K stage; the lead field is stored K-stage-major
+// so that a stage's tile is one contiguous block, and workgroups sharing a K range are placed
+// on one XCD so that its L2 serves the lead-field reuse across time tiles.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,7 +35,8 @@ namespace {
 
 constexpr int kTM = 128;  // time rows per workgroup
 constexpr int kTE = 64;   // electrodes per workgroup
-constexpr int kKS = 32;   // K per MFMA step
+constexpr int kKS = 32;   // K per MFMA step of the FP32 contraction
+constexpr int kK2 = 64;   // K of one LDS stage of the BF16 contraction (two 32-deep MFMA steps)
 
 __device__ __forceinline__ unsigned short bf16_rn(float x) {
   unsigned int b = __float_as_uint(x);
@@ -65,8 +69,8 @@ __device__ __forceinline__ double w_axis(const double* __restrict__ vox, const d
 __global__ void build_leadfield(int N, int E, int Kpad, const double* __restrict__ vox,
                                 const double* __restrict__ el, const double* __restrict__ c,
                                 const double* __restrict__ D, const int* __restrict__ nb,
-                                const int* __restrict__ perm, double inv2d, float* __restrict__ Wf,
-                                unsigned short* __restrict__ Whi, unsigned short* __restrict__ Wlo) {
+                                const int* __restrict__ perm, double inv2d, int Epad,
+                                float* __restrict__ Wf, unsigned short* __restrict__ Wt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int e = blockIdx.y;
   if (i >= Kpad) return;
@@ -87,17 +91,19 @@ __global__ void build_leadfield(int N, int E, int Kpad, const double* __restrict
   const float f = (float)acc;
   const unsigned short hi = bf16_rn(f);
   Wf[o] = f;
-  Whi[o] = hi;
-  Wlo[o] = bf16_rn(f - bf16_to_f(hi));
+  // BF16 planes in K-stage-major tiles [Kpad / 64][plane][Epad][64]: one K stage of every
+  // electrode and both planes is one contiguous block
+  const size_t tile = (size_t)(i / kK2) * 2 * Epad;
+  Wt[(tile + e) * kK2 + i % kK2] = hi;
+  Wt[(tile + Epad + e) * kK2 + i % kK2] = bf16_rn(f - bf16_to_f(hi));
 }
 
-template <int PREC>  // 0: bf16 hi/lo (3 MFMA), 1: fp32 MFMA, 2: bf16 single
-__global__ void __launch_bounds__(256) egm_gemm(const float* __restrict__ S, int64_t T,
-                                                int64_t stride, int N, int Kpad, int k_chunk,
-                                                const float* __restrict__ Wf,
-                                                const unsigned short* __restrict__ Whi,
-                                                const unsigned short* __restrict__ Wlo, int Epad,
-                                                float* __restrict__ ws) {
+// FP32 contraction on MFMA (v_mfma_f32_16x16x4f32), the accuracy reference of the BF16 form:
+// workgroup 4 waves = 128 time rows x 64 electrodes, samples read straight from memory.
+__global__ void __launch_bounds__(256) egm_gemm_f32(const float* __restrict__ S, int64_t T,
+                                                    int64_t stride, int N, int Kpad, int k_chunk,
+                                                    const float* __restrict__ Wf, int Epad,
+                                                    float* __restrict__ ws) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wt = wave & 1, we = wave >> 1;
@@ -134,38 +140,14 @@ __global__ void __launch_bounds__(256) egm_gemm(const float* __restrict__ S, int
 #pragma unroll
     for (int cc = 0; cc < 2; ++cc) {
       const size_t wo = (size_t)(e0 + 16 * cc + r16) * Kpad + kl;
-      if (PREC == 1) {
-        const float4 b0 = *(const float4*)(Wf + wo);
-        const float4 b1 = *(const float4*)(Wf + wo + 4);
-        const float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+      const float4 b0 = *(const float4*)(Wf + wo);
+      const float4 b1 = *(const float4*)(Wf + wo + 4);
+      const float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+      for (int r = 0; r < 4; ++r)
 #pragma unroll
-          for (int j = 0; j < 8; ++j)
-            acc[r][cc] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][j], bv[j], acc[r][cc], 0, 0, 0);
-      } else {
-        const u16x8 bh = *(const u16x8*)(Whi + wo);
-        const bf16x8 bhi = __builtin_bit_cast(bf16x8, bh);
-        bf16x8 blo;
-        if (PREC == 0) blo = __builtin_bit_cast(bf16x8, *(const u16x8*)(Wlo + wo));
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          u16x8 ah, al;
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            const unsigned short h = bf16_rn(a[r][j]);
-            ah[j] = h;
-            al[j] = bf16_rn(a[r][j] - bf16_to_f(h));
-          }
-          const bf16x8 ahi = __builtin_bit_cast(bf16x8, ah);
-          acc[r][cc] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, acc[r][cc], 0, 0, 0);
-          if (PREC == 0) {
-            const bf16x8 alo = __builtin_bit_cast(bf16x8, al);
-            acc[r][cc] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo, acc[r][cc], 0, 0, 0);
-            acc[r][cc] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi, acc[r][cc], 0, 0, 0);
-          }
-        }
-      }
+        for (int j = 0; j < 8; ++j)
+          acc[r][cc] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][j], bv[j], acc[r][cc], 0, 0, 0);
     }
   }
   // D tile layout: lane holds column (electrode) r16, rows (time) 4*q + j.
@@ -178,6 +160,193 @@ __global__ void __launch_bounds__(256) egm_gemm(const float* __restrict__ S, int
       for (int j = 0; j < 4; ++j) {
         const int64_t t = t0 + 16 * r + 4 * q + j;
         if (t < T) wsp[t * Epad + e0 + 16 * cc + r16] = acc[r][cc][j];
+      }
+}
+
+// BF16 split of a pair of samples for the MFMA A operand: hi = x rounded to BF16, lo = x - hi
+// (exact in FP32) rounded to BF16, both with the hardware round-to-nearest-even conversion.
+// hi + lo carries 17 significant bits of x.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split_pair(float x0, float x1, unsigned& hi, unsigned& lo) {
+  const bf16x2 h = {(__bf16)x0, (__bf16)x1};
+  hi = __builtin_bit_cast(unsigned, h);
+  const float d0 = x0 - __uint_as_float(hi << 16);
+  const float d1 = x1 - __uint_as_float(hi & 0xFFFF0000u);
+  const bf16x2 l = {(__bf16)d0, (__bf16)d1};
+  lo = __builtin_bit_cast(unsigned, l);
+}
+
+constexpr int kPitch = kK2 + 8;      // LDS row pitch in bf16 (16 B pad against bank conflicts)
+constexpr int kGemmThreads = 512;    // 8 waves: 2 across time x 4 across electrodes
+// 16-row time blocks per wave (workgroup: 32 RW time rows): 4 for up to 128 electrodes (the
+// sample stream dominates), 8 for wider electrode sets (halves the lead-field reads per flop)
+constexpr int rw_for(int ce) { return ce == 4 ? 8 : 4; }
+
+// egm partial (T x Epad, one K split), samples and lead field staged through LDS.
+// Workgroup = 8 waves: wave (tg, w) = (wave / 4, wave % 4) owns time rows
+// [tb + 16 RW tg, tb + 16 RW (tg + 1)) and electrodes [e0 + 16 CE w, e0 + 16 CE (w + 1)) as
+// RW x CE tiles of 16 x 16 with FP32 accumulators (workgroup: 32 RW time rows).  Per 64-deep K
+// stage the workgroup
+//   * reads its (32 RW) x 64 FP32 sample tile from HBM once (256 B per time row) and splits it
+//     into BF16 hi/lo planes in LDS;
+//   * copies its (64 CE) x 64 lead-field tile (BF16 hi/lo planes, L2/MALL-resident across
+//     time tiles) into LDS, once for both time groups;
+//   * runs every wave's MFMAs with both operands from LDS.
+// The next stage's tiles are loaded into registers while the current stage's MFMAs run.
+// LO: add the hi*lo and lo*hi products.
+// Operand layout of v_mfma_f32_16x16x32_bf16: lane (r16 = lane % 16, q = lane / 16) holds
+// A row r16 and B column r16, k = 8q .. 8q+7; the result lane holds column r16, rows 4q .. 4q+3.
+//
+// Tile order (1D grid, mt time tiles x ncols (electrode tile, K split) columns, the column count
+// padded to a multiple of 8): workgroups are dispatched to the 8 XCDs round-robin, so workgroup
+// id runs on XCD id % 8.  Slot id / 8 of XCD x walks the mt time tiles of column
+// 8 (slot / mt) + x, i.e. all time tiles of a column run on one XCD at about the same time
+// and read each lead-field stage once from HBM, then from that XCD's L2.
+template <int RW, int CE, bool LO>
+__global__ void __launch_bounds__(kGemmThreads) egm_gemm_lds(
+    const float* __restrict__ S, int64_t T, int64_t stride, int N, int Kpad, int k_chunk,
+    const unsigned short* __restrict__ Wt, int Epad, float* __restrict__ ws, int mt, int nt,
+    int ncols) {
+  constexpr int NP = LO ? 2 : 1;           // planes
+  constexpr int kTW = 32 * RW;             // time rows per workgroup
+  constexpr int EW = 64 * CE;              // electrodes per workgroup
+  constexpr int ACH = kTW * 4 / kGemmThreads;      // 16-float sample chunks per thread
+  constexpr int BCH = NP * EW * 8 / kGemmThreads;  // 16-byte lead-field chunks per thread
+  static_assert(ACH >= 1 && BCH >= 1, "staging chunks");
+  __shared__ unsigned lds_a[NP][kTW * kPitch / 2];
+  __shared__ unsigned lds_b[NP][EW * kPitch / 2];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int tg = wave >> 2, wv = wave & 3;
+  const int slot = (int)(blockIdx.x >> 3);
+  const int col = (slot / mt) * 8 + (int)(blockIdx.x & 7);
+  if (col >= ncols) return;  // padding column (whole workgroup)
+  const int split = col / nt;
+  const int64_t tb = (int64_t)(slot % mt) * kTW;
+  const int eb = (col % nt) * EW;
+  const int kb = split * k_chunk;
+  const int ke = min(N, kb + k_chunk);
+  const int r16 = lane & 15, q = lane >> 4;
+
+  // sample staging: chunk c = tid + 512 i covers row c / 4, k offset 16 (c % 4) of the stage
+  int srw[ACH], sk[ACH];
+  const float* srow[ACH];
+#pragma unroll
+  for (int i = 0; i < ACH; ++i) {
+    const int c = tid + kGemmThreads * i;
+    srw[i] = c >> 2;
+    sk[i] = 16 * (c & 3);
+    int64_t t = tb + srw[i];
+    if (t > T - 1) t = T - 1;
+    srow[i] = S + t * stride;
+  }
+  // lead-field staging: chunk j = tid + 512 i -> plane, electrode row, k offset 8 (j % 8)
+  int bpl[BCH], brw[BCH], bk[BCH];
+#pragma unroll
+  for (int i = 0; i < BCH; ++i) {
+    const int j = tid + kGemmThreads * i;
+    bpl[i] = j / (EW * 8);
+    brw[i] = (j / 8) % EW;
+    bk[i] = 8 * (j % 8);
+  }
+  float st[ACH][16];
+  uint4 bst[BCH];
+  auto load_stage = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < ACH; ++i) {
+      const int k = k0 + sk[i];
+      if (k + 16 <= ke) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float4 x;
+          __builtin_memcpy(&x, srow[i] + k + 4 * v, 16);
+          st[i][4 * v] = x.x;
+          st[i][4 * v + 1] = x.y;
+          st[i][4 * v + 2] = x.z;
+          st[i][4 * v + 3] = x.w;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) st[i][j] = k + j < ke ? srow[i][k + j] : 0.0f;
+      }
+    }
+    // lead field: zero-padded to Kpad, so whole stages are always in bounds; the stage's
+    // tile [plane][Epad][64] is contiguous
+    const unsigned short* tile = Wt + (size_t)(k0 / kK2) * 2 * Epad * kK2;
+#pragma unroll
+    for (int i = 0; i < BCH; ++i)
+      bst[i] = *(const uint4*)(tile + ((size_t)bpl[i] * Epad + eb + brw[i]) * kK2 + bk[i]);
+  };
+
+  f32x4 acc[RW][CE];
+#pragma unroll
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int c = 0; c < CE; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  if (kb < ke) load_stage(kb);
+  for (int k0 = kb; k0 < ke; k0 += kK2) {
+    __syncthreads();  // the previous stage's LDS reads are done
+    {
+#pragma unroll
+      for (int i = 0; i < ACH; ++i) {
+        unsigned h[8], l[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) split_pair(st[i][2 * j], st[i][2 * j + 1], h[j], l[j]);
+        const int o = (srw[i] * kPitch + sk[i]) / 2;
+        *(uint4*)&lds_a[0][o] = uint4{h[0], h[1], h[2], h[3]};
+        *(uint4*)&lds_a[0][o + 4] = uint4{h[4], h[5], h[6], h[7]};
+        if (LO) {
+          *(uint4*)&lds_a[NP - 1][o] = uint4{l[0], l[1], l[2], l[3]};
+          *(uint4*)&lds_a[NP - 1][o + 4] = uint4{l[4], l[5], l[6], l[7]};
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < BCH; ++i) *(uint4*)&lds_b[bpl[i]][(brw[i] * kPitch + bk[i]) / 2] = bst[i];
+    }
+    __syncthreads();
+    if (k0 + kK2 < ke) load_stage(k0 + kK2);  // next stage in flight during the MFMAs
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2) {
+      bf16x8 bh[CE], bl[CE];
+#pragma unroll
+      for (int c = 0; c < CE; ++c) {
+        const int o = ((16 * CE * wv + 16 * c + r16) * kPitch + 32 * s2 + 8 * q) / 2;
+        bh[c] = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_b[0][o]);
+        if (LO) bl[c] = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_b[NP - 1][o]);
+      }
+      // per time block: the products of all CE electrode tiles, pass by pass, so that
+      // consecutive MFMAs on one accumulator are CE apart
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        const int o = ((16 * RW * tg + 16 * r + r16) * kPitch + 32 * s2 + 8 * q) / 2;
+        const bf16x8 ah = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_a[0][o]);
+        bf16x8 al;
+        if (LO) al = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_a[NP - 1][o]);
+#pragma unroll
+        for (int c = 0; c < CE; ++c)
+          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[c], acc[r][c], 0, 0, 0);
+        if (LO) {
+#pragma unroll
+          for (int c = 0; c < CE; ++c)
+            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+          for (int c = 0; c < CE; ++c)
+            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[c], acc[r][c], 0, 0, 0);
+        }
+      }
+    }
+  }
+  float* wsp = ws + (size_t)split * T * Epad;
+  const int e0 = eb + 16 * CE * wv;
+#pragma unroll
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int c = 0; c < CE; ++c)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t t = tb + 16 * RW * tg + 16 * r + 4 * q + j;
+        if (t < T) wsp[t * Epad + e0 + 16 * c + r16] = acc[r][c][j];
       }
 }
 
@@ -243,9 +412,9 @@ __global__ void __launch_bounds__(kNormThreads) egm_normalize_k(float* __restric
 
 struct phs_egm {
   int N = 0, E = 0, Kpad = 0, Epad = 0;
+  int ce = 4;  // 16-electrode tiles per wave in egm_gemm_lds (workgroup: 64 CE electrodes)
   float* Wf = nullptr;
-  unsigned short* Whi = nullptr;
-  unsigned short* Wlo = nullptr;
+  unsigned short* Wt = nullptr;  // BF16 hi/lo planes, tiles [Kpad / 64][2][Epad][64]
   float* ws = nullptr;
   size_t ws_bytes = 0;
 };
@@ -285,7 +454,10 @@ int phs_egm_create(int32_t N, int32_t E, const double* voxel, const double* elec
   g->N = N;
   g->E = E;
   g->Kpad = (N + 63) / 64 * 64;
-  g->Epad = (E + kTE - 1) / kTE * kTE;
+  // workgroup width of the LDS-staged contraction: 4 waves x CE tiles of 16 electrodes;
+  // Epad is a whole number of workgroup columns
+  g->ce = E <= 64 ? 1 : (E <= 128 ? 2 : 4);
+  g->Epad = (E + 64 * g->ce - 1) / (64 * g->ce) * (64 * g->ce);
   const size_t nw = (size_t)g->Epad * g->Kpad;
   double *dv = nullptr, *de = nullptr, *dc = nullptr, *dD = nullptr;
   int *dnb = nullptr, *dperm = nullptr;
@@ -300,8 +472,8 @@ int phs_egm_create(int32_t N, int32_t E, const double* voxel, const double* elec
     if (code) phs_egm_destroy(g);
     return code ? phs::fail(code, msg) : PHS_OK;
   };
-  if ((e = hipMalloc(&g->Wf, 4 * nw)) || (e = hipMalloc(&g->Whi, 2 * nw)) ||
-      (e = hipMalloc(&g->Wlo, 2 * nw)) || (e = hipMalloc(&dv, 24 * (size_t)N)) ||
+  if ((e = hipMalloc(&g->Wf, 4 * nw)) || (e = hipMalloc(&g->Wt, 4 * nw)) ||
+      (e = hipMalloc(&dv, 24 * (size_t)N)) ||
       (e = hipMalloc(&de, 24 * (size_t)E)) || (e = hipMalloc(&dc, 8 * (size_t)N)) ||
       (D && (e = hipMalloc(&dD, 72 * (size_t)N))) || (e = hipMalloc(&dnb, 24 * (size_t)N)) ||
       (perm && (e = hipMalloc(&dperm, 4 * (size_t)N))))
@@ -315,7 +487,7 @@ int phs_egm_create(int32_t N, int32_t E, const double* voxel, const double* elec
     return cleanup(PHS_ERR_HIP, std::string("egm upload: ") + hipGetErrorString(e));
   const dim3 grid((g->Kpad + 255) / 256, g->Epad);
   build_leadfield<<<grid, 256, 0, s>>>(N, E, g->Kpad, dv, de, dc, dD, dnb, dperm,
-                                       1.0 / (2.0 * delta), g->Wf, g->Whi, g->Wlo);
+                                       1.0 / (2.0 * delta), g->Epad, g->Wf, g->Wt);
   if ((e = hipGetLastError()) || (e = hipStreamSynchronize(s)))
     return cleanup(PHS_ERR_HIP, std::string("build_leadfield: ") + hipGetErrorString(e));
   *handle = g;
@@ -328,11 +500,16 @@ int phs_egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, flo
     return phs::fail(PHS_ERR_ARG, "invalid electrogram run arguments");
   if (precision < 0 || precision > 2) return phs::fail(PHS_ERR_ARG, "precision must be 0, 1 or 2");
   hipStream_t s = (hipStream_t)stream;
-  const int mt = (int)((T + kTM - 1) / kTM);
-  const int nt = g->Epad / kTE;
-  const int ksteps = (g->N + kKS - 1) / kKS;
-  int splits = std::max(1, std::min(ksteps, (2048 + mt * nt - 1) / (mt * nt)));
-  const int k_chunk = (ksteps + splits - 1) / splits * kKS;
+  const bool lds = precision != 1;
+  const int tm = lds ? 32 * rw_for(g->ce) : kTM;
+  const int te = lds ? 64 * g->ce : kTE;
+  const int kstep = lds ? kK2 : kKS;
+  const int mt = (int)((T + tm - 1) / tm);
+  const int nt = g->Epad / te;
+  const int ksteps = (g->N + kstep - 1) / kstep;
+  // split K so that about 1024 workgroups (4 per CU) cover the device; chunks are whole K stages
+  int splits = std::max(1, std::min(ksteps, (1024 + mt * nt - 1) / (mt * nt)));
+  const int k_chunk = (ksteps + splits - 1) / splits * kstep;
   splits = (g->N + k_chunk - 1) / k_chunk;
   const size_t need = sizeof(float) * (size_t)splits * T * g->Epad;
   if (need > g->ws_bytes) {
@@ -342,16 +519,26 @@ int phs_egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, flo
     PHS_TRY_ALLOC(g->ws, need);
     g->ws_bytes = need;
   }
+  const int ncols = nt * splits;
+  const int ncols_pad = (ncols + 7) / 8 * 8;
   const dim3 grid(mt, nt, splits);
-  if (precision == 0)
-    egm_gemm<0><<<grid, 256, 0, s>>>(samples, T, stride, g->N, g->Kpad, k_chunk, g->Wf, g->Whi,
-                                     g->Wlo, g->Epad, g->ws);
-  else if (precision == 1)
-    egm_gemm<1><<<grid, 256, 0, s>>>(samples, T, stride, g->N, g->Kpad, k_chunk, g->Wf, g->Whi,
-                                     g->Wlo, g->Epad, g->ws);
-  else
-    egm_gemm<2><<<grid, 256, 0, s>>>(samples, T, stride, g->N, g->Kpad, k_chunk, g->Wf, g->Whi,
-                                     g->Wlo, g->Epad, g->ws);
+  if (precision == 1) {
+    egm_gemm_f32<<<grid, 256, 0, s>>>(samples, T, stride, g->N, g->Kpad, k_chunk, g->Wf,
+                                      g->Epad, g->ws);
+  } else {
+    auto launch = [&](auto kern) {
+      kern<<<dim3(mt * ncols_pad), kGemmThreads, 0, s>>>(samples, T, stride, g->N, g->Kpad,
+                                                         k_chunk, g->Wt, g->Epad, g->ws, mt, nt,
+                                                         ncols);
+    };
+    const bool lo = precision == 0;
+    if (g->ce == 4)
+      lo ? launch(egm_gemm_lds<rw_for(4), 4, true>) : launch(egm_gemm_lds<rw_for(4), 4, false>);
+    else if (g->ce == 2)
+      lo ? launch(egm_gemm_lds<rw_for(2), 2, true>) : launch(egm_gemm_lds<rw_for(2), 2, false>);
+    else
+      lo ? launch(egm_gemm_lds<rw_for(1), 1, true>) : launch(egm_gemm_lds<rw_for(1), 1, false>);
+  }
   const int64_t n = T * g->E;
   egm_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g->ws, splits, T, g->Epad, g->E, egm);
   PHS_HIP_CHECK(hipGetLastError());
@@ -361,8 +548,7 @@ int phs_egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, flo
 int phs_egm_destroy(phs_egm* g) {
   if (!g) return PHS_OK;
   hipFree(g->Wf);
-  hipFree(g->Whi);
-  hipFree(g->Wlo);
+  hipFree(g->Wt);
   hipFree(g->ws);
   delete g;
   return PHS_OK;

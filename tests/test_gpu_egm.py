@@ -14,7 +14,7 @@ def _rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("T,E", [(1, 1), (37, 5), (130, 64), (257, 70)])
+@pytest.mark.parametrize("T,E", [(1, 1), (37, 5), (130, 64), (257, 70), (70, 200), (300, 300)])
 def test_egm_random_samples(cuda, T, E):
     from oracle import egm_oracle
     from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
