@@ -145,11 +145,14 @@ struct Shape {
 };
 
 // Per-brick, per-lane stencil addressing (everything derived from the brick record).
+// Every gathered value of a lane without an occupied cross-brick neighbour is the lane's own
+// value (its gather address is its own index), so a valid lane's neighbour value is its
+// in-brick neighbour's where that cell is occupied and the gathered value otherwise: an
+// absent neighbour then contributes v - v = 0.
 template <int BX, int BY, int BZ>
 struct Stencil {
   using SH = Shape<BX, BY, BZ>;
-  uint64_t pin[6];  // lanes whose d-neighbour is an occupied cell of this brick
-  uint64_t pcr[6];  // lanes whose d-neighbour is an occupied cell of the neighbour brick
+  uint64_t pin[6];  // lanes whose d-neighbour is an occupied cell of this brick (valid lanes)
   int gaddr[6];     // gather address of the cross neighbour (own index where none)
   int idx;          // compact index (the brick's first voxel on empty lanes)
   bool valid;
@@ -163,13 +166,15 @@ struct Stencil {
     for (int d = 0; d < 6; ++d) {
       const int st = SH::step(d), k = SH::wrap(d);
       const uint64_t IN = SH::inb(d);
-      pin[d] = ((d & 1) ? (br.mask << st) : (br.mask >> st)) & IN & br.mask;
+      // bits of empty lanes are don't-cares: their results are never stored
+      pin[d] = ((d & 1) ? (br.mask << st) : (br.mask >> st)) & IN;
       // the cell of the neighbour brick adjacent to lane l is l - k (+ face) or l + k (- face);
-      // nbase already counts the neighbour's cells below k for a - face
+      // nstart already counts the neighbour's cells below k for a - face.  Empty lanes keep
+      // their own (in-range) index.
       const uint64_t sh = (d & 1) ? (br.nmask[d] >> k) : (br.nmask[d] << k);
-      pcr[d] = sh & ~IN & br.mask;
+      const uint64_t pcr = sh & ~IN & br.mask;
       const int cross = br.nstart[d] + mbcnt64(sh);
-      gaddr[d] = lane_bit(pcr[d]) ? cross : idx;
+      gaddr[d] = lane_bit(pcr) ? cross : idx;
     }
   }
 
@@ -208,20 +213,19 @@ struct Stencil {
     }
   }
 
-  // sum over present face neighbours of (v_m - v_n) in type T; own value v, cross values g
-  // scaled by sc.  All six lane exchanges are issued before any is consumed.
+  // sum over present face neighbours of (v_m - v_n) in type T; own value v, gathered values g
+  // scaled by sc (v must equal the lane's own gathered value times sc, bit for bit).  All six
+  // lane exchanges are issued before any is consumed.
   template <typename T, typename V>
   __device__ __forceinline__ T sum(V v, const V (&g)[6], V sc, int lane) const {
     const V pv[6] = {exch<0>(v, lane), exch<1>(v, lane), exch<2>(v, lane),
                      exch<3>(v, lane), exch<4>(v, lane), exch<5>(v, lane)};
-    T s = (T)0;
+    T s;
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
-      // two lane-mask selects (pin and pcr are disjoint); absent neighbours give m = v
-      V m = g[d] * sc;
-      m = lane_bit(pcr[d]) ? m : v;
-      m = lane_bit(pin[d]) ? pv[d] : m;
-      s += (T)m - (T)v;
+      const V m = lane_bit(pin[d]) ? pv[d] : g[d] * sc;
+      const T t = (T)m - (T)v;
+      s = d == 0 ? t : s + t;
     }
     return s;
   }
@@ -417,16 +421,15 @@ __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotRec& S
 }
 
 // One Chebyshev sweep d_{k+1} = a_k d_k + b_k d_{k-1} + g_k (r - A d_k), A d = d - alpha sum.
-// cur_kind 1: d_k = r / theta (k = 1).  prev_kind 0: d_{k-1} = 0; 1: r / theta; 2: prev[].
-// GUESS: warm start of the correction (see guess_of); the final modes add it back and keep
-// the step's total correction in D1 (and the previous one in D2) for the next step.
-template <int BX, int BY, int BZ, int MODE, int NB, int GUESS>
+// FIRST (k = 1): d_k = r / theta, read from R0.  prev_kind 0: d_{k-1} = 0; 1: r / theta;
+// 2: prev[].  GUESS: warm start of the correction (see guess_of); the final modes add it back
+// and keep the step's total correction in D1 (and the previous one in D2) for the next step.
+template <int BX, int BY, int BZ, int MODE, int NB, int GUESS, bool FIRST>
 __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, const float* cur,
-                                                  int cur_kind, const float* prev, int prev_kind,
-                                                  float* out) {
+                                                  const float* prev, int prev_kind, float* out) {
   const int b0 = first_brick<NB>(A.xcd);
   const int lane = lane_id();
-  const float* src = cur_kind ? A.R0 : cur;
+  const float* src = FIRST ? A.R0 : cur;
   Stencil<BX, BY, BZ> st[NB];
   SlotView sv[NB];
   bool on[NB];
@@ -457,7 +460,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
   for (int j = 0; j < NB; ++j) {
     if (!on[j] || sv[j].skip) continue;
     const float inv_theta = sv[j].inv_theta;
-    const float sc = cur_kind ? inv_theta : 1.0f;
+    const float sc = FIRST ? inv_theta : 1.0f;
     const float d = st[j].valid ? dn[j] * sc : 0.0f;
     const float s = st[j].template sum<float>(d, g6[j], sc, lane);
     if (!st[j].valid || !sv[j].live) continue;

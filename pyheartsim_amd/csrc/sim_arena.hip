@@ -164,23 +164,24 @@ int grid_for(const phs_arena* a) {
 // Launches of one time step for brick shape (BX, BY, BZ) and NB bricks per wavefront.
 template <int BX, int BY, int BZ, int NB, int GUESS>
 struct Step {
+  template <int MODE>
+  static void sweep_m(int grid, hipStream_t s, const ArenaDev& A, int k, int K, const float* cur,
+                      const float* prev, int prev_kind, float* out) {
+    if (k == 1)
+      k_sweep<BX, BY, BZ, MODE, NB, GUESS, true><<<grid, kBlock, 0, s>>>(A, k, K, cur, prev,
+                                                                         prev_kind, out);
+    else
+      k_sweep<BX, BY, BZ, MODE, NB, GUESS, false><<<grid, kBlock, 0, s>>>(A, k, K, cur, prev,
+                                                                          prev_kind, out);
+  }
   static void sweep(int mode, int grid, hipStream_t s, const ArenaDev& A, int k, int K,
-                    const float* cur, int cur_kind, const float* prev, int prev_kind, float* out) {
+                    const float* cur, const float* prev, int prev_kind, float* out) {
     switch (mode) {
-      case MODE_MID:
-        k_sweep<BX, BY, BZ, MODE_MID, NB, GUESS><<<grid, kBlock, 0, s>>>(A, k, K, cur, cur_kind, prev, prev_kind, out);
-        break;
-      case MODE_FINAL_X:
-        k_sweep<BX, BY, BZ, MODE_FINAL_X, NB, GUESS><<<grid, kBlock, 0, s>>>(A, k, K, cur, cur_kind, prev, prev_kind, out);
-        break;
-      case MODE_DACC_SET:
-        k_sweep<BX, BY, BZ, MODE_DACC_SET, NB, GUESS><<<grid, kBlock, 0, s>>>(A, k, K, cur, cur_kind, prev, prev_kind, out);
-        break;
-      case MODE_DACC_ADD:
-        k_sweep<BX, BY, BZ, MODE_DACC_ADD, NB, GUESS><<<grid, kBlock, 0, s>>>(A, k, K, cur, cur_kind, prev, prev_kind, out);
-        break;
-      default:
-        k_sweep<BX, BY, BZ, MODE_DACC_ADD_X, NB, GUESS><<<grid, kBlock, 0, s>>>(A, k, K, cur, cur_kind, prev, prev_kind, out);
+      case MODE_MID: sweep_m<MODE_MID>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
+      case MODE_FINAL_X: sweep_m<MODE_FINAL_X>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
+      case MODE_DACC_SET: sweep_m<MODE_DACC_SET>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
+      case MODE_DACC_ADD: sweep_m<MODE_DACC_ADD>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
+      default: sweep_m<MODE_DACC_ADD_X>(grid, s, A, k, K, cur, prev, prev_kind, out);
     }
   }
 
@@ -198,10 +199,9 @@ struct Step {
       const int K = per_slot ? m : 0;
       for (int k = 1; k < m; ++k) {
         const float* cur = k == 1 ? nullptr : bufs[k % 2];
-        const int cur_kind = k == 1 ? 1 : 0;
         const float* prev = k <= 2 ? nullptr : bufs[(k - 1) % 2];
         const int prev_kind = k == 1 ? 0 : (k == 2 ? 1 : 2);
-        sweep(k < m - 1 ? MODE_MID : last_mode, grid, s, A, k, K, cur, cur_kind, prev, prev_kind,
+        sweep(k < m - 1 ? MODE_MID : last_mode, grid, s, A, k, K, cur, prev, prev_kind,
               bufs[(k + 1) % 2]);
       }
     };
@@ -235,7 +235,8 @@ struct Step {
     for (int i = 0; i < reps; ++i) k_rhs<BX, BY, BZ, NB, GUESS><<<grid, kBlock, 0, s>>>(A);
     hipEventRecord(ev[2], s);
     for (int i = 0; i < reps; ++i)
-      k_sweep<BX, BY, BZ, MODE_MID, NB, GUESS><<<grid, kBlock, 0, s>>>(A, 3, 0, a->DA, 0, a->DB, 2, a->DB);
+      k_sweep<BX, BY, BZ, MODE_MID, NB, GUESS, false><<<grid, kBlock, 0, s>>>(A, 3, 0, a->DA, a->DB, 2,
+                                                                           a->DB);
     hipEventRecord(ev[3], s);
     hipError_t r = hipEventSynchronize(ev[3]);
     for (int i = 0; i < 3; ++i) {
