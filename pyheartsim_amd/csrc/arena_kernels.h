@@ -38,7 +38,8 @@ struct SlotRec {
   int64_t step;       // steps done
   int64_t n_steps;
   int64_t voxel_off;
-  int64_t cur_row;    // sample row written by this step, -1 = none
+  int64_t cur_off;    // element offset of the sample row written by this step in the sample
+                      // buffers (row modulo the ring length, times n_voxel), -1 = none
   const uint8_t* sched;
   float* su;
   float* sh;
@@ -412,9 +413,8 @@ __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotRec& S
   } else if (!now && above) {
     A.VF[idx] = (uint8_t)(f & ~PHS_VF_ABOVE);
   }
-  if (S.cur_row >= 0) {
-    const int64_t row = S.ring > 0 ? S.cur_row % S.ring : S.cur_row;
-    const int64_t o = row * S.n_voxel + local;
+  if (S.cur_off >= 0) {
+    const int64_t o = S.cur_off + local;
     if (S.su) S.su[o] = x;
     if (S.sh) S.sh[o] = A.H[idx];
   }
@@ -546,7 +546,7 @@ __global__ void k_advance(SlotRec* slots, int n) {
   } else {
     row = t;
   }
-  S.cur_row = row < S.n_samples ? row : -1;
+  S.cur_off = row >= 0 && row < S.n_samples ? (S.ring > 0 ? row % S.ring : row) * S.n_voxel : -1;
 }
 
 __global__ void k_deactivate(SlotRec* slots, int slot) { slots[slot].live = 0; }

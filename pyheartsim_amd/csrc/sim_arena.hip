@@ -563,7 +563,7 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
   sr.step = 0;
   sr.n_steps = d->n_steps;
   sr.cur_sched = d->n_steps > 0 ? d->schedule[0] : 0;
-  sr.cur_row = d->n_samples > 0 ? 0 : -1;
+  sr.cur_off = d->n_samples > 0 ? 0 : -1;  // step 0 writes sample row 0
   sr.voxel_off = vo;
   sr.su = out ? (float*)out->samples_u : nullptr;
   sr.sh = out ? (float*)out->samples_h : nullptr;
