@@ -37,6 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BRICK_RECORD_BYTES = 192  # sizeof(BrickRec): masks, gather bases and slot of one 64-cell brick
 BF16_PEAK_TFLOPS = 2500.0
 
 
@@ -166,7 +167,7 @@ def run_ragged(args, torch, runtime):
                                                 preps[i].n_steps)))
         del plain
     log(f"ragged validation: {val}")
-    bytes_sweep = cnt["voxels"] * 16 + cnt["vf_voxels"] + cnt["bricks"] * 96
+    bytes_sweep = cnt["voxels"] * 16 + cnt["vf_voxels"] + cnt["bricks"] * BRICK_RECORD_BYTES
     return dict(workload="ragged", sims=len(preps), voxels=sum(ns), voxel_steps_per_s=value,
                 sweeps=arena.sweeps(),
                 ms_per_step=1e3 * t / ks, steps=ks, kernels_ms=prof,
@@ -378,9 +379,9 @@ def main():
     prof = arena.profile(50)
     vf = cnt["vf_voxels"]
     # algorithmic bytes of one Chebyshev sweep launch: read d_k, d_{k-1}, r (FP32) and write
-    # d_{k+1} per voxel, the flag byte of scar/temporary-block simulations, and the 96-byte
+    # d_{k+1} per voxel, the flag byte of scar/temporary-block simulations, and the 192-byte
     # brick records (neighbour values gathered across brick faces are re-reads, not counted)
-    bytes_sweep = cnt["voxels"] * 16 + vf + cnt["bricks"] * 96
+    bytes_sweep = cnt["voxels"] * 16 + vf + cnt["bricks"] * BRICK_RECORD_BYTES
     achieved = bytes_sweep / (prof["sweep"] * 1e-3) / 1e9
     log(f"batched: {value:.4g} voxel-steps/s, {ms_step:.4f} ms/step, kernels {prof}, counts {cnt}")
 

@@ -1,11 +1,12 @@
 // Device code of the bricked FP32 arena (included by sim_arena.hip only).
 //
 // A brick is 64 lattice cells handled by one wavefront (lane l <-> cell
-// (l % BX, (l / BX) % BY, l / (BX*BY))).  The brick record (96 B, one scalar load) holds the
-// brick's occupancy mask and compact start and those of its six face-neighbour bricks, so
-// every lane's loads depend on that one scalar load only, and all of them are issued before
-// any value is used.  Neighbour presence is computed on the wave-uniform 64-bit masks with
-// scalar shifts and used as lane masks (inverse ballot); compact indices come from mbcnt.
+// (l % BX, (l / BX) % BY, l / (BX*BY))).  The brick record (192 B, scalar loads) holds the
+// brick's occupancy mask and compact start, its per-direction neighbour-presence lane masks
+// and the shifted masks and gather bases of its six face-neighbour bricks, all precomputed on
+// the host, so every lane's loads depend on that one record only, and all of them are issued
+// before any value is used.  Presence masks are used as lane masks (inverse ballot);
+// compact indices come from mbcnt.
 // In-brick neighbour values move between lanes with ds_bpermute (__shfl); cross-brick
 // values are gathered (lanes without a cross neighbour re-load their own value, which hits
 // the same cache line and keeps the code branch-free).
@@ -23,16 +24,23 @@ constexpr int kWaves = 4;
 constexpr int kBlock = 64 * kWaves;
 constexpr int kMaxSweeps = 64;
 
+// Brick record, built on the host when a simulation is added; everything a wavefront needs
+// to address its brick's stencil is here, so the kernels spend no scalar work on deriving it.
+// Directions d: 0 +x, 1 -x, 2 +y, 3 -y, 4 +z, 5 -z.
 struct BrickRec {
   uint64_t mask;       // occupied cells
   int32_t start;       // compact index of the first occupied cell
   int32_t slot;        // simulation slot, -1 = unused brick
-  uint64_t nmask[6];   // face-neighbour bricks' masks (0 = no brick)
+  uint64_t pin[6];     // lanes whose d-neighbour is an occupied cell of this brick
+  uint64_t pcr[6];     // occupied lanes whose d-neighbour is an occupied cell of the
+                       // d-neighbour brick
+  uint64_t sh[6];      // the d-neighbour brick's mask moved so that bit l is the cell
+                       // adjacent to lane l across the face (its mbcnt is the gather rank)
   int32_t nstart[6];   // gather base of each face-neighbour brick: its start, plus, for a
                        // - face, the number of its cells below the face-adjacent offset
   int32_t pad[2];
 };
-static_assert(sizeof(BrickRec) == 96, "brick record layout");
+static_assert(sizeof(BrickRec) == 192, "brick record layout");
 
 struct SlotRec {
   int64_t step;       // steps done
@@ -146,15 +154,17 @@ struct Shape {
 };
 
 // Per-brick, per-lane stencil addressing (everything derived from the brick record).
-// Every gathered value of a lane without an occupied cross-brick neighbour is the lane's own
-// value (its gather address is its own index), so a valid lane's neighbour value is its
-// in-brick neighbour's where that cell is occupied and the gathered value otherwise: an
-// absent neighbour then contributes v - v = 0.
+// Cross-brick values are gathered once per axis: a lane lies on at most one face of an axis
+// whose brick extent is >= 2, so one load brings its + or - cross neighbour (or, on lanes with
+// neither, its own value).  Axes of extent 1 (both faces on every lane) load both.
 template <int BX, int BY, int BZ>
 struct Stencil {
   using SH = Shape<BX, BY, BZ>;
-  uint64_t pin[6];  // lanes whose d-neighbour is an occupied cell of this brick (valid lanes)
-  int gaddr[6];     // gather address of the cross neighbour (own index where none)
+  static constexpr bool both(int ax) { return SH::B[ax] == 1; }
+  uint64_t pin[6];  // lanes whose d-neighbour is an occupied cell of this brick
+  uint64_t pcr[6];  // occupied lanes whose d-neighbour is an occupied cell across the face
+  int gaddr[6];     // gather addresses: [2 ax] per axis (+ face, - face or own index), and
+                    // [2 ax + 1] the - face of an extent-1 axis
   int idx;          // compact index (the brick's first voxel on empty lanes)
   bool valid;
 
@@ -165,24 +175,30 @@ struct Stencil {
     idx = br.start + (valid ? own : 0);
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
-      const int st = SH::step(d), k = SH::wrap(d);
-      const uint64_t IN = SH::inb(d);
-      // bits of empty lanes are don't-cares: their results are never stored
-      pin[d] = ((d & 1) ? (br.mask << st) : (br.mask >> st)) & IN;
-      // the cell of the neighbour brick adjacent to lane l is l - k (+ face) or l + k (- face);
-      // nstart already counts the neighbour's cells below k for a - face.  Empty lanes keep
-      // their own (in-range) index.
-      const uint64_t sh = (d & 1) ? (br.nmask[d] >> k) : (br.nmask[d] << k);
-      const uint64_t pcr = sh & ~IN & br.mask;
-      const int cross = br.nstart[d] + mbcnt64(sh);
-      gaddr[d] = lane_bit(pcr) ? cross : idx;
+      pin[d] = br.pin[d];
+      pcr[d] = br.pcr[d];
+    }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      const int cp = br.nstart[2 * ax] + mbcnt64(br.sh[2 * ax]);
+      const int cm = br.nstart[2 * ax + 1] + mbcnt64(br.sh[2 * ax + 1]);
+      if (both(ax)) {
+        gaddr[2 * ax] = lane_bit(pcr[2 * ax]) ? cp : idx;
+        gaddr[2 * ax + 1] = lane_bit(pcr[2 * ax + 1]) ? cm : idx;
+      } else {
+        gaddr[2 * ax] = lane_bit(pcr[2 * ax]) ? cp : (lane_bit(pcr[2 * ax + 1]) ? cm : idx);
+      }
     }
   }
 
+  // g[d]: the gathered value for direction d (shared by the two directions of an axis).
   template <typename V>
   __device__ __forceinline__ void gather(const V* __restrict__ src, V (&g)[6]) const {
 #pragma unroll
-    for (int d = 0; d < 6; ++d) g[d] = ld(src, gaddr[d]);
+    for (int ax = 0; ax < 3; ++ax) {
+      g[2 * ax] = ld(src, gaddr[2 * ax]);
+      g[2 * ax + 1] = both(ax) ? ld(src, gaddr[2 * ax + 1]) : g[2 * ax];
+    }
   }
 
   // Value of the in-brick neighbour lane in direction D: a DPP row shift when the axis
@@ -215,8 +231,8 @@ struct Stencil {
   }
 
   // sum over present face neighbours of (v_m - v_n) in type T; own value v, gathered values g
-  // scaled by sc (v must equal the lane's own gathered value times sc, bit for bit).  All six
-  // lane exchanges are issued before any is consumed.
+  // scaled by sc.  Absent neighbours give m = v.  All six lane exchanges are issued before
+  // any is consumed.
   template <typename T, typename V>
   __device__ __forceinline__ T sum(V v, const V (&g)[6], V sc, int lane) const {
     const V pv[6] = {exch<0>(v, lane), exch<1>(v, lane), exch<2>(v, lane),
@@ -224,7 +240,8 @@ struct Stencil {
     T s;
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
-      const V m = lane_bit(pin[d]) ? pv[d] : g[d] * sc;
+      V m = lane_bit(pcr[d]) ? g[d] * sc : v;
+      m = lane_bit(pin[d]) ? pv[d] : m;
       const T t = (T)m - (T)v;
       s = d == 0 ? t : s + t;
     }

@@ -468,21 +468,40 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
   bool needs_vf = false;
   double amax_deg = 0.0;
   const double a0 = d->dt * d->c / (2.0 * d->delta * d->delta);
+  const int bd[3] = {a->cfg.brick[0], a->cfg.brick[1], a->cfg.brick[2]};
+  const int stride[3] = {1, bd[0], bd[0] * bd[1]};
+  uint64_t inb[6];  // lanes whose d-neighbour cell lies in the same brick
+  for (int k = 0; k < 6; ++k) {
+    inb[k] = 0;
+    for (int l = 0; l < 64; ++l) {
+      const int c[3] = {l % bd[0], (l / bd[0]) % bd[1], l / (bd[0] * bd[1])};
+      const int ax = k >> 1;
+      if ((k & 1) ? c[ax] > 0 : c[ax] < bd[ax] - 1) inb[k] |= 1ULL << l;
+    }
+  }
+  auto shl = [](uint64_t m, int n) { return n >= 64 ? 0ULL : m << n; };
+  auto shr = [](uint64_t m, int n) { return n >= 64 ? 0ULL : m >> n; };
   for (int32_t b = 0; b < P.n_bricks; ++b) {
     BrickRec r;
-    r.mask = P.mask[b];
+    const uint64_t mask = P.mask[b];
+    r.mask = mask;
     r.start = (int32_t)(vo + P.start[b]);
     r.slot = slot;
     for (int k = 0; k < 6; ++k) {
+      const int ax = k >> 1;
+      const int st = stride[ax];
+      const int wrap = (bd[ax] - 1) * stride[ax];
       const int32_t nb = P.nbb[(size_t)b * 6 + k];
-      r.nmask[k] = nb >= 0 ? P.mask[nb] : 0;
+      const uint64_t nmask = nb >= 0 ? P.mask[nb] : 0;
+      // + direction: the neighbour of lane l is lane l + st (in the brick) or the neighbour
+      // brick's cell l - wrap; - direction: l - st, or the cell l + wrap
+      r.pin[k] = ((k & 1) ? shl(mask, st) : shr(mask, st)) & inb[k] & mask;
+      r.sh[k] = (k & 1) ? shr(nmask, wrap) : shl(nmask, wrap);
+      r.pcr[k] = r.sh[k] & ~inb[k] & mask;
       int32_t below = 0;
       if (nb >= 0 && (k & 1)) {
         // - face: the gather rank is counted from the face-adjacent offset (B-1)*stride
-        const int bd[3] = {a->cfg.brick[0], a->cfg.brick[1], a->cfg.brick[2]};
-        const int stride[3] = {1, bd[0], bd[0] * bd[1]};
-        const int off = (bd[k >> 1] - 1) * stride[k >> 1];
-        below = off > 0 ? __builtin_popcountll(P.mask[nb] & ((1ULL << off) - 1ULL)) : 0;
+        below = wrap > 0 ? __builtin_popcountll(nmask & ((1ULL << wrap) - 1ULL)) : 0;
       }
       r.nstart[k] = nb >= 0 ? (int32_t)(vo + P.start[nb] + below) : 0;
     }

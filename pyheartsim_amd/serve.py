@@ -52,7 +52,7 @@ class Request:
         kpad = (n + 63) // 64 * 64
         epad = (e + 63) // 64 * 64
         return (n * 33            # arena state per voxel (u, h, r, 2 sweep buffers, warm start, flags, alpha)
-                + n * 96 // 40    # brick records (~40 voxels per brick)
+                + n * 192 // 40   # brick records (~40 voxels per brick)
                 + 4 * ring * n    # sample ring
                 + 8 * epad * kpad  # lead field (fp32 + bf16 hi/lo)
                 + 28 * n          # activation times (7 beats)
