@@ -41,6 +41,26 @@ BRICK_RECORD_BYTES = 192  # sizeof(BrickRec): masks, gather bases and slot of on
 BF16_PEAK_TFLOPS = 2500.0
 
 
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "arena_sweep_pmc_shell_ms_x16.md")
+
+
+def profiled_traffic(cnt):
+    """HBM bytes (read + write) of one middle-sweep launch, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated on a device copy), when that summary was
+    measured on an arena of the same voxels and bricks; else None."""
+    try:
+        with open(PROFILE_TRAFFIC) as f:
+            for line in f:
+                if line.startswith("traffic:"):
+                    rec = json.loads(line[len("traffic:"):])
+                    if (rec["voxels"] == cnt["voxels"] and rec["bricks"] == cnt["bricks"]
+                            and rec["record_bytes"] == BRICK_RECORD_BYTES):
+                        return rec["sweep_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -264,6 +284,8 @@ def main():
                     help="run a device copy of this size first (HBM counter calibration)")
     ap.add_argument("--graph-steps", type=int, default=50)
     ap.add_argument("--no-validate", action="store_true")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the batched leg (short profiler runs)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -300,71 +322,75 @@ def main():
         del src, dst
     res = {}
 
-    # ---- plain FP64, one simulation
-    plain = runtime.PlainSim(prep, tol=1e-15, samples=not args.no_validate, sample_h=False,
-                             sample_fp64=False)
-    sweeps_plain = plain.info()[0]
-    t = timed(torch, plain.run, K, W, 200)
-    res["plain"] = dict(voxel_steps_per_s=N * K / t, ms_per_step=1e3 * t / K, sweeps=sweeps_plain)
-    log(f"plain fp64: {res['plain']}")
-    u_mid, h_mid = plain.state()
-    u_mid = u_mid.cpu().numpy()
-    h_mid = h_mid.cpu().numpy()
-    step_mid = W + K
-
-    # ---- chosen form, one simulation
-    single = runtime.Arena(N, 1, solver)
-    slot = single.add(prep, samples=not args.no_validate)
-    t = timed(torch, lambda n: single.step(n, graph=True), (K // G) * G, (W // G) * G, G)
     ks = (K // G) * G
-    res["single"] = dict(voxel_steps_per_s=N * ks / t, ms_per_step=1e3 * t / ks)
-    log(f"chosen single: {res['single']}")
+    u_mid = h_mid = step_mid = None
+    if not args.only_batched:
+        # ---- plain FP64, one simulation
+        plain = runtime.PlainSim(prep, tol=1e-15, samples=not args.no_validate, sample_h=False,
+                                 sample_fp64=False)
+        sweeps_plain = plain.info()[0]
+        t = timed(torch, plain.run, K, W, 200)
+        res["plain"] = dict(voxel_steps_per_s=N * K / t, ms_per_step=1e3 * t / K, sweeps=sweeps_plain)
+        log(f"plain fp64: {res['plain']}")
+        u_mid, h_mid = plain.state()
+        u_mid = u_mid.cpu().numpy()
+        h_mid = h_mid.cpu().numpy()
+        step_mid = W + K
 
-    # ---- validation: run both to the end of the 2000 ms simulation
-    if not args.no_validate:
-        t0 = time.perf_counter()
-        plain.run()
-        while single.steps_done(slot) < S_full:
-            single.step(G, graph=True)
-        torch.cuda.synchronize()
-        log(f"validation runs finished in {time.perf_counter()-t0:.1f}s")
-        su_c, _, lat_c = single.outputs(slot)
-        lat_p = plain.lat.cpu().numpy()
-        lat_cn = lat_c.cpu().numpy()
-        le = lat_error(lat_cn, lat_p, S_full)
-        T = prep.n_saved()
-        vm = float((su_c[:T] - plain.su[:T]).abs().max().item())
-        res["validate"] = dict(lat_err_steps=le, max_abs_dVm=vm, beats=int((lat_p[:, 0] >= 0).sum()))
-        log(f"validation: {res['validate']}")
-        # electrogram: chosen (BF16 hi/lo on compact-order samples) vs FP32 MFMA on plain samples
-        el = work["electrodes"]
-        c = work["hm"]["c_voxel"]
-        lf_c = LeadField(el, g["voxel"], None, c, g["Delta"], g["neighbor_id_2d"],
-                         perm=single.slots[slot].perm)
-        lf_p = LeadField(el, g["voxel"], None, c, g["Delta"], g["neighbor_id_2d"])
-        samples_c = single.slots[slot].su[:T]
-        e_ref = lf_p.run(plain.su[:T], precision="fp32")
-        e_c = lf_c.run(samples_c, precision="bf16x3")
-        torch.cuda.synchronize()
-        rel = float(((e_c - e_ref).norm() / e_ref.norm()).item())
-        st = torch.cuda.current_stream()
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        ev0.record(st)
-        reps = 5
-        for _ in range(reps):
-            lf_c.run(samples_c, precision="bf16x3", normalize=False)
-        ev1.record(st)
-        torch.cuda.synchronize()
-        ms = ev0.elapsed_time(ev1) / reps
-        E = el.shape[0]
-        useful = 2.0 * E * T * N
-        res["egm"] = dict(rel_l2=rel, ms=ms, electrodes=E, samples=T,
-                          useful_tflops=useful / ms / 1e9,
-                          mfma_issued_tflops=3 * useful / ms / 1e9,
-                          reference_equivalent_tflops=2.0 * E * T * 3 * N / ms / 1e9)
-        log(f"egm: {res['egm']}")
-        del plain, single
-        torch.cuda.empty_cache()
+        # ---- chosen form, one simulation
+        single = runtime.Arena(N, 1, solver)
+        slot = single.add(prep, samples=not args.no_validate)
+        t = timed(torch, lambda n: single.step(n, graph=True), (K // G) * G, (W // G) * G, G)
+        res["single"] = dict(voxel_steps_per_s=N * ks / t, ms_per_step=1e3 * t / ks)
+        log(f"chosen single: {res['single']}")
+
+        # ---- validation: run both to the end of the 2000 ms simulation
+        if not args.no_validate:
+            t0 = time.perf_counter()
+            plain.run()
+            while single.steps_done(slot) < S_full:
+                single.step(G, graph=True)
+            torch.cuda.synchronize()
+            log(f"validation runs finished in {time.perf_counter()-t0:.1f}s")
+            su_c, _, lat_c = single.outputs(slot)
+            lat_p = plain.lat.cpu().numpy()
+            lat_cn = lat_c.cpu().numpy()
+            le = lat_error(lat_cn, lat_p, S_full)
+            T = prep.n_saved()
+            vm = float((su_c[:T] - plain.su[:T]).abs().max().item())
+            res["validate"] = dict(lat_err_steps=le, max_abs_dVm=vm, beats=int((lat_p[:, 0] >= 0).sum()))
+            log(f"validation: {res['validate']}")
+            # electrogram: chosen (BF16 hi/lo on compact-order samples) vs FP32 MFMA on plain samples
+            el = work["electrodes"]
+            c = work["hm"]["c_voxel"]
+            lf_c = LeadField(el, g["voxel"], None, c, g["Delta"], g["neighbor_id_2d"],
+                             perm=single.slots[slot].perm)
+            lf_p = LeadField(el, g["voxel"], None, c, g["Delta"], g["neighbor_id_2d"])
+            samples_c = single.slots[slot].su[:T]
+            e_ref = lf_p.run(plain.su[:T], precision="fp32")
+            e_c = lf_c.run(samples_c, precision="bf16x3")
+            torch.cuda.synchronize()
+            rel = float(((e_c - e_ref).norm() / e_ref.norm()).item())
+            st = torch.cuda.current_stream()
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record(st)
+            reps = 5
+            for _ in range(reps):
+                lf_c.run(samples_c, precision="bf16x3", normalize=False)
+            ev1.record(st)
+            torch.cuda.synchronize()
+            ms = ev0.elapsed_time(ev1) / reps
+            E = el.shape[0]
+            useful = 2.0 * E * T * N
+            res["egm"] = dict(rel_l2=rel, ms=ms, electrodes=E, samples=T,
+                              useful_tflops=useful / ms / 1e9,
+                              mfma_issued_tflops=3 * useful / ms / 1e9,
+                              reference_equivalent_tflops=2.0 * E * T * 3 * N / ms / 1e9)
+            log(f"egm: {res['egm']}")
+            del plain, single
+            torch.cuda.empty_cache()
+        else:
+            del plain, single
 
     # ---- chosen form, batched
     B = len(preps)
@@ -390,7 +416,7 @@ def main():
 
     # ---- served form: open-loop Poisson stream, continuous batching
     serve_res = None
-    if args.serve_requests > 0:
+    if args.serve_requests > 0 and not args.only_batched:
         from pyheartsim_amd import serve
         t0 = time.perf_counter()
         reqs = serve.make_requests(args.serve_requests, args.qps, seed=11)
@@ -408,7 +434,7 @@ def main():
         log(f"serve: {serve_res}")
 
     cpu = None
-    if args.cpu_seconds > 0:
+    if args.cpu_seconds > 0 and u_mid is not None:
         cpu = cpu_baseline(work, u_mid, h_mid, step_mid, args.cpu_seconds)
         log(f"cpu: {cpu}")
 
@@ -432,9 +458,10 @@ def main():
             "n_voxel": N, "batch": B, "steps_per_sim": S_full, "sweeps": sweeps_used,
             "guess": args.guess, "brick": [4, 4, 4], "graph_steps": G,
         },
-        "speedup_vs_plain_fp64": value / res["plain"]["voxel_steps_per_s"],
-        "plain_fp64": res["plain"],
-        "single": res["single"],
+        "speedup_vs_plain_fp64": (value / res["plain"]["voxel_steps_per_s"]
+                                  if "plain" in res else None),
+        "plain_fp64": res.get("plain"),
+        "single": res.get("single"),
         "validate": res.get("validate"),
         "egm": res.get("egm"),
         "latency_ms": ({"p50": 1e3 * serve_res["p50_s"], "p99": 1e3 * serve_res["p99_s"]}
@@ -443,7 +470,7 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": "arena_sweep (Chebyshev sweep, FP32)",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(cnt),
             "bytes_per_launch": bytes_sweep, "avg_launch_ms": prof["sweep"],
             "kernels_ms": prof,
         },
