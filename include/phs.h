@@ -189,7 +189,9 @@ int phs_arena_slot_info(phs_arena* a, int32_t slot, int64_t* steps_done, int32_t
                         int32_t* n_bricks, int64_t* voxel_offset);
 /* Compact (brick) order -> caller order map of a slot: perm[i] = caller id of voxel i. */
 int phs_arena_perm(phs_arena* a, int32_t slot, int32_t* perm_host);
-/* Device-to-device copy of a slot's current state in caller order (float[n_voxel]). */
+/* Device-to-device copy of a slot's current state in caller order (float[n_voxel]).  The
+ * next step's reaction runs at the end of each step, so between steps u is that reaction's
+ * u* (and h its h); after a simulation's last step it is the final state. */
 int phs_arena_state(phs_arena* a, int32_t slot, float* u, float* h, void* stream);
 /* Chebyshev sweeps and spectral bounds used for a slot. */
 int phs_arena_slot_bounds(phs_arena* a, int32_t slot, double* lambda_min, double* lambda_max);

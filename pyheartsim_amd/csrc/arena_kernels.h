@@ -55,6 +55,8 @@ struct SlotRec {
   int32_t n_voxel;
   int32_t live;       // resident and step < n_steps
   int32_t cur_sched;  // schedule byte of this step
+  int32_t next_sched; // schedule byte of the next step (its reaction runs in this step's
+                      // last sweep), 0 after the last step
   int32_t model;
   int32_t nspm;
   int32_t n_samples;
@@ -302,24 +304,59 @@ __device__ __forceinline__ T alpha_of(const ArenaDev& A, const SlotRec& S, int i
   return a;
 }
 
-__device__ __forceinline__ void rhs32(const SlotRec& S, float u, float h, float J, float& du,
+// Cell-model right-hand side (simulation/simulation_gpu.py:27-43) in FP32, branch-free:
+// MODEL 0 Mitchell-Schaeffer (p = 1/tau_open, 1/tau_close, 1/tau_in, 1/tau_out, v_gate),
+// MODEL 1 Aliev-Panfilov (p = k, a, eps0, mu1, mu2).
+template <int MODEL>
+__device__ __forceinline__ void rhs32(const float (&p)[5], float u, float h, float J, float& du,
                                       float& dh) {
-  if (S.model == 0) {
-    du = h * u * u * (1.0f - u) * S.p[2] - u * S.p[3] + J;
-    dh = (u < S.p[4]) ? (1.0f - h) * S.p[0] : -h * S.p[1];
+  if (MODEL == 0) {
+    du = h * u * u * (1.0f - u) * p[2] - u * p[3] + J;
+    const float open = (1.0f - h) * p[0], close = -h * p[1];
+    dh = u < p[4] ? open : close;
   } else {
-    const float k = S.p[0], a = S.p[1];
+    const float k = p[0], a = p[1];
     du = -k * u * (u - a) * (u - 1.0f) - u * h + J;
-    dh = (S.p[2] + S.p[3] * h * __builtin_amdgcn_rcpf(u + S.p[4])) * (-h - k * u * (u - a - 1.0f));
+    dh = (p[2] + p[3] * h * __builtin_amdgcn_rcpf(u + p[4])) * (-h - k * u * (u - a - 1.0f));
   }
 }
 
-// RK4 (simulation/simulation_gpu.py:71-130) in FP32, NB bricks per wavefront.  Loads are
-// issued on the brick record alone (a finished simulation's bricks still address resident
-// data); the slot's liveness is checked before anything is stored.
+template <int MODEL>
+__device__ __forceinline__ void rk4_model(const float (&p)[5], float dt, float J, float u, float h,
+                                          float& uo, float& ho) {
+  float k1u, k1h, k2u, k2h, k3u, k3h, k4u, k4h;
+  rhs32<MODEL>(p, u, h, J, k1u, k1h);
+  rhs32<MODEL>(p, u + 0.5f * dt * k1u, h + 0.5f * dt * k1h, J, k2u, k2h);
+  rhs32<MODEL>(p, u + 0.5f * dt * k2u, h + 0.5f * dt * k2h, J, k3u, k3h);
+  rhs32<MODEL>(p, u + dt * k3u, h + dt * k3h, J, k4u, k4h);
+  const float s6 = dt * (1.0f / 6.0f);
+  uo = u + s6 * (k1u + 2.0f * k2u + 2.0f * k3u + k4u);
+  ho = h + s6 * (k1h + 2.0f * k2h + 2.0f * k3h + k4h);
+}
+
+// One RK4 step of the cell model (simulation/simulation_gpu.py:71-130) in FP32; the stimulus
+// J is 1 on the S1 (S2) sites while the schedule byte sc has the S1 (S2) bit.  The model is
+// wave-uniform: one scalar branch per call.
+__device__ __forceinline__ void rk4(const SlotRec& S, int sc, uint8_t f, float u, float h,
+                                    float& uo, float& ho) {
+  const float J = (((sc & PHS_SCHED_S1) && (f & PHS_VF_S1)) ||
+                   ((sc & PHS_SCHED_S2) && (f & PHS_VF_S2)))
+                      ? 1.0f
+                      : 0.0f;
+  const float p[5] = {S.p[0], S.p[1], S.p[2], S.p[3], S.p[4]};
+  if (S.model == 0)
+    rk4_model<0>(p, S.dt, J, u, h, uo, ho);
+  else
+    rk4_model<1>(p, S.dt, J, u, h, uo, ho);
+}
+
+// The reaction of a step on its own: the first step of a newly added simulation (bricks
+// [b_lo, b_hi) only), and the timing of the reaction in isolation.  Every later step's
+// reaction runs inside the previous step's last sweep.  Loads are issued on the brick
+// record alone; the slot's liveness is checked before anything is stored.
 template <int NB>
-__global__ void __launch_bounds__(kBlock) k_reaction(ArenaDev A) {
-  const int b0 = first_brick<NB>(A.xcd);
+__global__ void __launch_bounds__(kBlock) k_reaction(ArenaDev A, int b_lo, int b_hi) {
+  const int b0 = first_brick<NB>(A.xcd) + b_lo;
   float u[NB], h[NB];
   uint8_t f[NB];
   int idx[NB];
@@ -327,7 +364,7 @@ __global__ void __launch_bounds__(kBlock) k_reaction(ArenaDev A) {
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int b = b0 + j;
-    on[j] = b < A.n_bricks && A.bricks[b].slot >= 0;
+    on[j] = b < b_hi && A.bricks[b].slot >= 0;
     if (on[j]) {
       const BrickRec& br = A.bricks[b];
       valid[j] = lane_bit(br.mask);
@@ -342,20 +379,10 @@ __global__ void __launch_bounds__(kBlock) k_reaction(ArenaDev A) {
     if (!on[j]) continue;
     const SlotRec& S = A.slots[A.bricks[b0 + j].slot];
     if (!S.live || !valid[j]) continue;
-    const int sc = S.cur_sched;
-    const float J = (((sc & PHS_SCHED_S1) && (f[j] & PHS_VF_S1)) ||
-                     ((sc & PHS_SCHED_S2) && (f[j] & PHS_VF_S2)))
-                        ? 1.0f
-                        : 0.0f;
-    const float dt = S.dt;
-    float k1u, k1h, k2u, k2h, k3u, k3h, k4u, k4h;
-    rhs32(S, u[j], h[j], J, k1u, k1h);
-    rhs32(S, u[j] + 0.5f * dt * k1u, h[j] + 0.5f * dt * k1h, J, k2u, k2h);
-    rhs32(S, u[j] + 0.5f * dt * k2u, h[j] + 0.5f * dt * k2h, J, k3u, k3h);
-    rhs32(S, u[j] + dt * k3u, h[j] + dt * k3h, J, k4u, k4h);
-    const float s6 = dt * (1.0f / 6.0f);
-    stv(A.UO, idx[j], u[j] + s6 * (k1u + 2.0f * k2u + 2.0f * k3u + k4u));
-    stv(A.HO, idx[j], h[j] + s6 * (k1h + 2.0f * k2h + 2.0f * k3h + k4h));
+    float uo, ho;
+    rk4(S, S.cur_sched, f[j], u[j], h[j], uo, ho);
+    stv(A.UO, idx[j], uo);
+    stv(A.HO, idx[j], ho);
   }
 }
 
@@ -415,10 +442,12 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
   }
 }
 
-// Activation time, samples and state of the last pass of a step.
+// Last pass of a step: activation time and samples of the post-step state x, then the next
+// step's reaction, so that U holds u* of the next step between steps (x itself after a
+// simulation's last step).
 __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotRec& S, int idx, float x) {
-  A.U[idx] = x;
   const uint8_t f = A.VF[idx];
+  const float hv = A.H[idx];
   const bool now = x >= S.lat_thr;
   const bool above = (f & PHS_VF_ABOVE) != 0;
   const int64_t local = (int64_t)idx - S.voxel_off;
@@ -433,7 +462,15 @@ __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotRec& S
   if (S.cur_off >= 0) {
     const int64_t o = S.cur_off + local;
     if (S.su) S.su[o] = x;
-    if (S.sh) S.sh[o] = A.H[idx];
+    if (S.sh) S.sh[o] = hv;
+  }
+  if (S.step + 1 < S.n_steps) {
+    float uo, ho;
+    rk4(S, S.next_sched, f, x, hv, uo, ho);
+    A.U[idx] = uo;
+    A.H[idx] = ho;
+  } else {
+    A.U[idx] = x;
   }
 }
 
@@ -557,6 +594,7 @@ __global__ void k_advance(SlotRec* slots, int n) {
     return;
   }
   S.cur_sched = S.sched[t];
+  S.next_sched = t + 1 < S.n_steps ? S.sched[t + 1] : 0;
   int64_t row = -1;
   if (S.nspm > 0) {
     if (t % S.nspm == 0) row = t / S.nspm;

@@ -190,7 +190,7 @@ struct Step {
     const int grid = grid_for(a);
     const int grid1 = grid_for_nb1(a);
     const phs_arena_config& c = a->cfg;
-    k_reaction<NB><<<grid, kBlock, 0, s>>>(A);
+    // the step's reaction already ran in the previous step's last sweep (or at add)
     k_rhs<BX, BY, BZ, NB, GUESS><<<grid, kBlock, 0, s>>>(A);
     float* bufs[2] = {a->DA, a->DB};
     // One Chebyshev solve of m sweeps on the right-hand side held in R0; per_slot: each
@@ -230,7 +230,8 @@ struct Step {
     R.UO = a->DA;
     R.HO = a->DB;
     hipEventRecord(ev[0], s);
-    for (int i = 0; i < reps; ++i) k_reaction<NB><<<grid, kBlock, 0, s>>>(R);
+    for (int i = 0; i < reps; ++i)
+      k_reaction<NB><<<grid, kBlock, 0, s>>>(R, 0, (int)a->brick_high);
     hipEventRecord(ev[1], s);
     for (int i = 0; i < reps; ++i) k_rhs<BX, BY, BZ, NB, GUESS><<<grid, kBlock, 0, s>>>(A);
     hipEventRecord(ev[2], s);
@@ -246,6 +247,15 @@ struct Step {
     }
     for (auto& e : ev) hipEventDestroy(e);
     return r;
+  }
+
+  // Reaction of step 0 for the bricks [b_lo, b_hi) of a newly added simulation.
+  static void react_range(phs_arena* a, int b_lo, int b_hi, hipStream_t s) {
+    ArenaDev A = dev_view(a);
+    A.xcd = 0;
+    const int per = kWaves * NB;
+    const int grid = (b_hi - b_lo + per - 1) / per;
+    if (grid > 0) k_reaction<NB><<<grid, kBlock, 0, s>>>(A, b_lo, b_hi);
   }
 
   static void diag(phs_arena* a, int slot, hipStream_t s) {
@@ -294,6 +304,11 @@ hipError_t profile_kernels(phs_arena* a, int reps, double* ms, hipStream_t s) {
   hipError_t r = hipSuccess;
   dispatch(a, [&](auto st) { r = decltype(st)::profile(a, reps, ms, s); });
   return r;
+}
+
+hipError_t launch_react_range(phs_arena* a, int b_lo, int b_hi, hipStream_t s) {
+  dispatch(a, [&](auto st) { decltype(st)::react_range(a, b_lo, b_hi, s); });
+  return hipGetLastError();
 }
 
 void launch_diag(phs_arena* a, int slot, hipStream_t s) {
@@ -582,6 +597,7 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
   sr.step = 0;
   sr.n_steps = d->n_steps;
   sr.cur_sched = d->n_steps > 0 ? d->schedule[0] : 0;
+  sr.next_sched = d->n_steps > 1 ? d->schedule[1] : 0;
   sr.cur_off = d->n_samples > 0 ? 0 : -1;  // step 0 writes sample row 0
   sr.voxel_off = vo;
   sr.su = out ? (float*)out->samples_u : nullptr;
@@ -629,6 +645,7 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
       (a->D1 && (e = hipMemsetAsync(a->D1 + vo, 0, sz_f, s))) ||
       (a->D2 && (e = hipMemsetAsync(a->D2 + vo, 0, sz_f, s))) ||
       (e = hipMemcpyAsync(a->slots + slot, hs, sz_sl, hipMemcpyHostToDevice, s)) ||
+      (e = launch_react_range(a, (int)bo, (int)(bo + P.n_bricks), s)) ||
       (e = hipEventCreateWithFlags(&h.upload_done, hipEventDisableTiming)) ||
       (e = hipEventRecord(h.upload_done, s)))
     return undo(PHS_ERR_HIP, std::string("slot upload: ") + hipGetErrorString(e));

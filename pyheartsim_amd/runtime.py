@@ -372,6 +372,8 @@ class Arena:
         return dict(voxels=v.value, bricks=b.value, vf_voxels=f.value, launch_bricks=lb.value)
 
     def state(self, slot: int):
+        """(u, h) of a slot in caller order.  Between steps u is the next step's u* (its
+        reaction runs at the end of each step); after the last step, the final state."""
         torch = self.torch
         n = self.slots[slot].n_voxel
         u = torch.empty(n, dtype=torch.float32, device="cuda")
