@@ -218,11 +218,11 @@ def run_precision(args, torch, runtime):
                         lambda_max=plain.info()[2])
     ref = plain.lat.cpu().numpy()
     del plain
-    # automatic sweep count (stability + accuracy rule), without and with one FP64-residual
-    # refinement pass of 8 sweeps
-    for refine in (0, 1):
+    # automatic sweep count (stability + accuracy rule) without refinement, and with three
+    # FP64-residual refinement passes of 12 sweeps (CN residual <= 1e-10)
+    for refine, rsw in ((0, 8), (3, 12)):
         cfg = runtime.SolverConfig(sweeps=args.sweeps, guess=args.guess, refine=refine,
-                                   refine_sweeps=8, diag=True)
+                                   refine_sweeps=rsw, diag=True)
         arena = runtime.Arena(N, 1, cfg)
         slot = arena.add(prep, samples=False)
         lmax = arena.bounds(slot)[1]
@@ -236,7 +236,8 @@ def run_precision(args, torch, runtime):
         rate = N * prep.n_steps / dt
         out[f"refine{refine}"] = dict(
             voxel_steps_per_s=rate, speedup_vs_plain=rate / plain_rate, sweeps=arena.sweeps(),
-            stable_sweeps=runtime.stable_sweeps(lmax, args.guess, refine, 8)[0],
+            refine_sweeps=rsw if refine else 0,
+            stable_sweeps=runtime.stable_sweeps(lmax, args.guess, refine, rsw)[0],
             lambda_max=lmax, cn_rel_residual=res, lat_err_steps=lat_error(lat.cpu().numpy(), ref, prep.n_steps))
         log(f"precision refine={refine}: {out[f'refine{refine}']}")
         arena.close()

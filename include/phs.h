@@ -245,6 +245,12 @@ int phs_egm_destroy(phs_egm* handle);
  * (simulation/unipolar_electrogram_equation_parts.py:118-124). */
 int phs_egm_normalize(float* egm, int64_t n_time, int32_t n_electrode, double typical,
                       void* stream);
+/* Electrode activation times (replaces utility/lat_map.py:20-27 compute_electrode_lat):
+ * lat[e] = argmax_t -(egm[t+1][e] - egm[t][e]) (first index on ties) minus its minimum over
+ * electrodes.  egm device float [n_time][n_electrode] (n_time >= 2); lat device int32
+ * [n_electrode].  No allocation; stream-ordered. */
+int phs_egm_electrode_lat(const float* egm, int64_t n_time, int32_t n_electrode, int32_t* lat,
+                          void* stream);
 
 #ifdef __cplusplus
 }

@@ -12,12 +12,14 @@ Restates, in FP64:
                           = simulation/unipolar_electrogram_gpu.py:54-69:
                           c_v / l^3 * ((D grad u) . l_vec), summed over voxels
   * normalisation         simulation/unipolar_electrogram_equation_parts.py:118-124
+  * electrode LAT         utility/lat_map.py:20-27
 Layout follows the GPU API (simulation/unipolar_electrogram_gpu.py:71-152):
 action_potential is (T x N), the result is (T x E).
 
 Pinned by tests/test_oracle_egm.py: a one-voxel dipole closed form, the
 zero gradient on one-sided axes, linearity, the normalisation invariant,
-and a brute-force triple loop on tiny inputs.
+a brute-force triple loop on tiny inputs, and hand-built electrode signals
+with known steepest-downstroke indices (ties, shift to 0).
 """
 from __future__ import annotations
 
@@ -99,3 +101,13 @@ def electrogram_bruteforce(electrode_xyz, voxel, D, c_voxel, action_potential, D
                 l = max(float(np.sqrt(lvec @ lvec)), 1.0)
                 out[t, e] += c_voxel[v] / l ** 3 * float(Dg @ lvec)
     return out
+
+
+def electrode_lat(electrode_signal):
+    """Electrode activation time of utility/lat_map.py:20-27: per electrode (axis 1) the time
+    index of the steepest downstroke, argmax_t -(s[t+1] - s[t]) (first index on ties, as
+    NumPy's argmax), shifted so that the earliest electrode is 0."""
+    s = np.asarray(electrode_signal)
+    steep = -np.diff(s, axis=0)
+    idx = np.argmax(steep, axis=0)
+    return idx - np.min(idx)

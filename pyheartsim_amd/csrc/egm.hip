@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <vector>
 
 #include "phs_internal.h"
@@ -408,6 +409,58 @@ __global__ void __launch_bounds__(kNormThreads) egm_normalize_k(float* __restric
   for (int64_t i = tid; i < n; i += kNormThreads) egm[i] = (float)(egm[i] * scale);
 }
 
+// Electrode activation time from the electrogram (utility/lat_map.py:20-27): per electrode
+// the first time index of the steepest downstroke, argmax_t -(egm[t+1] - egm[t]), then
+// shifted so that the earliest electrode is 0.  One workgroup per electrode; ties go to the
+// lower index (NumPy argmax).
+constexpr int kLatThreads = 256;
+__global__ void __launch_bounds__(kLatThreads) egm_downstroke_k(const float* __restrict__ egm,
+                                                                int64_t T, int E,
+                                                                int32_t* __restrict__ idx) {
+  __shared__ float sv[kLatThreads];
+  __shared__ int64_t si[kLatThreads];
+  const int e = blockIdx.x, tid = threadIdx.x;
+  float best = -INFINITY;
+  int64_t bi = T;  // sentinel above every index
+  for (int64_t t = tid; t + 1 < T; t += kLatThreads) {
+    const float v = -(egm[(t + 1) * E + e] - egm[t * E + e]);
+    if (v > best) {  // strided: a later t of the same thread never wins a tie
+      best = v;
+      bi = t;
+    }
+  }
+  sv[tid] = best;
+  si[tid] = bi;
+  __syncthreads();
+  for (int o = kLatThreads / 2; o > 0; o >>= 1) {
+    if (tid < o) {
+      const float v = sv[tid + o];
+      const int64_t i = si[tid + o];
+      if (v > sv[tid] || (v == sv[tid] && i < si[tid])) {
+        sv[tid] = v;
+        si[tid] = i;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) idx[e] = si[0] < T ? (int32_t)si[0] : 0;
+}
+
+__global__ void __launch_bounds__(kLatThreads) lat_shift_k(int32_t* __restrict__ lat, int E) {
+  __shared__ int32_t sm[kLatThreads];
+  const int tid = threadIdx.x;
+  int32_t m = INT32_MAX;
+  for (int e = tid; e < E; e += kLatThreads) m = min(m, lat[e]);
+  sm[tid] = m;
+  __syncthreads();
+  for (int o = kLatThreads / 2; o > 0; o >>= 1) {
+    if (tid < o) sm[tid] = min(sm[tid], sm[tid + o]);
+    __syncthreads();
+  }
+  const int32_t lo = sm[0];
+  for (int e = tid; e < E; e += kLatThreads) lat[e] -= lo;
+}
+
 }  // namespace
 
 struct phs_egm {
@@ -558,6 +611,15 @@ int phs_egm_normalize(float* egm, int64_t T, int32_t E, double typical, void* st
   if (!egm || T <= 0 || E <= 0) return phs::fail(PHS_ERR_ARG, "invalid normalize arguments");
   if (E > kNormMaxE) return phs::fail(PHS_ERR_ARG, "normalize supports at most 4096 electrodes");
   egm_normalize_k<<<1, kNormThreads, 0, (hipStream_t)stream>>>(egm, T, E, typical);
+  PHS_HIP_CHECK(hipGetLastError());
+  return PHS_OK;
+}
+
+int phs_egm_electrode_lat(const float* egm, int64_t T, int32_t E, int32_t* lat, void* stream) {
+  if (!egm || !lat || T < 2 || E <= 0) return phs::fail(PHS_ERR_ARG, "invalid electrode LAT arguments");
+  hipStream_t s = (hipStream_t)stream;
+  egm_downstroke_k<<<E, kLatThreads, 0, s>>>(egm, T, E, lat);
+  lat_shift_k<<<1, kLatThreads, 0, s>>>(lat, E);
   PHS_HIP_CHECK(hipGetLastError());
   return PHS_OK;
 }

@@ -53,3 +53,34 @@ def test_egm_dropin_normalised(cuda):
                                            g["neighbor_id_2d"])
     assert out.shape == ref.shape and out.dtype == np.float32
     assert _rel_l2(out, ref) <= 1e-3
+
+
+@pytest.mark.parametrize("T,E", [(2, 1), (12, 3), (2000, 64), (777, 300)])
+def test_electrode_lat_matches_oracle(cuda, T, E):
+    from oracle import egm_oracle
+    from pyheartsim_amd.utility.lat_map import compute_electrode_lat
+    rng = np.random.default_rng(T + E)
+    s = np.cumsum(rng.standard_normal((T, E)), axis=0).astype(np.float32)
+    s[T // 2:, 0] = s[T // 2, 0]          # flat tails and exact ties
+    if T > 4:
+        s[:, -1] = np.repeat([3.0, 2.0, 1.0, 0.0], (T + 3) // 4)[:T]
+    got = compute_electrode_lat(s)
+    want = egm_oracle.electrode_lat(s)
+    assert got.dtype == np.int64
+    assert np.array_equal(got, want)
+
+
+def test_electrode_lat_of_simulated_egm(cuda):
+    from oracle import egm_oracle
+    from pyheartsim_amd.simulation import unipolar_electrogram_gpu
+    from pyheartsim_amd.utility.lat_map import compute_electrode_lat
+    w = small_shell()
+    g = w["geometry"]
+    N = g["voxel"].shape[0]
+    T = 120
+    x = g["voxel"][:, 0]
+    ap = 1.0 / (1.0 + np.exp(-(np.arange(T)[:, None] * 0.2 - (x[None, :] + 9.0))))
+    egm = unipolar_electrogram_gpu.compute(w["electrodes"], g["voxel"], None, np.full(N, 4.0), ap,
+                                           g["Delta"], g["neighbor_id_2d"], device_out=True)
+    got = compute_electrode_lat(egm)
+    assert np.array_equal(got, egm_oracle.electrode_lat(egm.cpu().numpy()))

@@ -75,3 +75,20 @@ def test_dropin_helpers_agree_with_oracle():
         assert np.array_equal(a, b)
     x = rng.normal(size=(7, 5))
     assert np.allclose(ep.normalize_electrogram_magnitude(x), eo.normalize(x))
+
+
+def test_electrode_lat_known_answers():
+    # electrode 0: one steep fall between t=5 and t=6; electrode 1: two equal falls at t=2 and
+    # t=7 (first index wins); electrode 2: a ramp down whose steepest step is t=8 -> t=9
+    T = 12
+    s = np.zeros((T, 3))
+    s[:6, 0] = 1.0
+    s[:3, 1] = 2.0
+    s[3:8, 1] = 1.0
+    s[:, 2] = -np.arange(T) * 0.1
+    s[9:, 2] -= 0.5
+    lat = eo.electrode_lat(s)
+    assert lat.tolist() == [5 - 2, 2 - 2, 8 - 2]
+    # a single electrode is always 0; constant signals pick t = 0
+    assert eo.electrode_lat(s[:, :1]).tolist() == [0]
+    assert eo.electrode_lat(np.ones((5, 4))).tolist() == [0, 0, 0, 0]
