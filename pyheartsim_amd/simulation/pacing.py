@@ -21,6 +21,63 @@ SCHED_S2 = 2
 SCHED_TEMP_BLOCK = 4
 
 
+def find_out_s2_pacing_voxel_ids_for_rotor_arrhythmia(s1, geometry_data):
+    """S2 site for a rotor set-up (simulation/pacing.py:16-100).
+
+    Candidates are the voxels 12-32 mm from the S1 voxel.  They are grouped into connected
+    clusters over the 18-neighbour table, by depth-first search from each not yet visited
+    candidate in increasing id order.  The largest cluster (the first found among equals)
+    is taken.  If it holds more than 300 voxels, S2 is the first 300 voxels of a
+    breadth-first search from its first voxel, neighbours in table column order.
+    Otherwise S2 is that single first voxel.  Membership tests use boolean masks instead of
+    the reference's list scans; the visiting order is the same.
+    """
+    voxel = np.asarray(geometry_data["voxel"], dtype=np.float64)
+    nb = np.asarray(geometry_data["neighbor_id_2d"])
+    n = voxel.shape[0]
+    d = np.sqrt(np.sum((voxel - voxel[s1, :]) ** 2, axis=1))
+    cand = np.nonzero((d >= 12) & (d <= 12 + 20))[0]
+    is_cand = np.zeros(n, dtype=bool)
+    is_cand[cand] = True
+    visited = np.zeros(n, dtype=bool)
+    best = []
+    for root in cand:
+        if visited[root]:
+            continue
+        cluster = []
+        stack = [int(root)]
+        visited[root] = True
+        while stack:
+            cur = stack.pop()
+            cluster.append(cur)
+            for m in nb[cur]:
+                if m >= 0 and is_cand[m] and not visited[m]:
+                    visited[m] = True
+                    stack.append(int(m))
+        if len(cluster) > len(best):
+            best = cluster
+    first = best[0]
+    n_threshold = 300
+    if len(best) <= n_threshold:
+        return first
+    in_best = np.zeros(n, dtype=bool)
+    in_best[best] = True
+    seen = np.zeros(n, dtype=bool)
+    seen[first] = True
+    queue = [first]
+    head = 0
+    out = []
+    while head < len(queue) and len(out) < n_threshold:
+        cur = queue[head]
+        head += 1
+        out.append(cur)
+        for m in nb[cur]:
+            if m >= 0 and in_best[m] and not seen[m]:
+                seen[m] = True
+                queue.append(int(m))
+    return out
+
+
 def assign_pacing_parameters(arrhythmia_parameters, arrhythmia_flag, n_voxel, neighbor_id_2d,
                              simulation_parameters):
     """Reference simulation/pacing.py:102-134 (same 7-tuple)."""
