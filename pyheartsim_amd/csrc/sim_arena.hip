@@ -117,6 +117,7 @@ struct phs_arena {
   hipGraph_t graph = nullptr;
   int32_t g_steps = -1;
   int64_t g_grid = -1;
+  int64_t g_high = -1;     // brick_high captured in the graph's kernel arguments
   int32_t g_sweeps = -1;
   int32_t sweeps_eff = 2;  // sweeps of the first CN solve in the launched sequence
 };
@@ -699,7 +700,8 @@ int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream) {
   if (a->brick_high == 0) return PHS_OK;
   hipStream_t s = (hipStream_t)stream;
   const int64_t grid = grid_for(a);
-  if (!a->gexec || a->g_steps != n_steps || a->g_grid != grid || a->g_sweeps != a->sweeps_eff) {
+  if (!a->gexec || a->g_steps != n_steps || a->g_grid != grid || a->g_high != a->brick_high ||
+      a->g_sweeps != a->sweeps_eff) {
     if (a->gexec) hipGraphExecDestroy(a->gexec);
     if (a->graph) hipGraphDestroy(a->graph);
     a->gexec = nullptr;
@@ -713,6 +715,7 @@ int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream) {
     PHS_HIP_CHECK(hipGraphInstantiate(&a->gexec, g, nullptr, nullptr, 0));
     a->g_steps = n_steps;
     a->g_grid = grid;
+    a->g_high = a->brick_high;
     a->g_sweeps = a->sweeps_eff;
   }
   PHS_HIP_CHECK(hipGraphLaunch(a->gexec, s));

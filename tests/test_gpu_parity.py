@@ -180,3 +180,62 @@ def test_fixed_sweeps_below_stability_limit_rejected(cuda):
     a = runtime.Arena(p.n_voxel, 1, runtime.SolverConfig(sweeps=4))
     with pytest.raises(_lib.PhsError, match="unstable"):
         a.add(p)
+
+
+def _run_pair_with_removal(solver, wa, wb):
+    """Two simulations packed; the first is removed after 60 steps (its bricks stay in the
+    table without a slot), the second runs to its end with graph replays."""
+    from pyheartsim_amd import runtime
+    pa, pb = _prep(wa), _prep(wb)
+    a = runtime.Arena(pa.n_voxel + pb.n_voxel, 2, solver)
+    sa = a.add(pa)
+    sb = a.add(pb)
+    a.step(60)
+    a.remove(sa)
+    while a.steps_done(sb) < pb.n_steps:
+        a.step(50, graph=True)
+    u, _, lat = a.outputs(sb)
+    out = (u.cpu().numpy(), lat.cpu().numpy())
+    a.close()
+    return out
+
+
+@pytest.mark.parametrize("brick,guess,refine", [((4, 4, 4), 1, 0), ((8, 8, 1), 2, 0),
+                                                ((4, 4, 4), 0, 1)])
+def test_two_bricks_per_wavefront_equals_one(cuda, brick, guess, refine):
+    """Two bricks per wavefront (bricks_per_wave = 2) compute every voxel with the same
+    operations in the same order as one brick per wavefront: bitwise equal, also with
+    removed simulations' bricks (no slot) in the table."""
+    from pyheartsim_amd import runtime
+    wa = small_shell(t_final=10.0, seed=3)
+    wb = small_shell(t_final=20.0, seed=5, semi_axes=(8.0, 7.0, 6.0), n_scar=1, temp_block=True)
+    res = {}
+    for bpw in (1, 2):
+        solver = runtime.SolverConfig(brick=brick, guess=guess, refine=refine, bricks_per_wave=bpw)
+        res[bpw] = _run_pair_with_removal(solver, wa, wb)
+    assert np.array_equal(res[2][0], res[1][0])
+    assert np.array_equal(res[2][1], res[1][1])
+
+
+@pytest.mark.parametrize("bpw", [1, 2])
+def test_graph_recaptured_when_bricks_added_within_grid(cuda, bpw):
+    """A simulation admitted between graph replays is stepped by the next replay also when
+    the launch grid does not change (a few bricks added below the grid's capacity)."""
+    from pyheartsim_amd import runtime
+    pa, pb = _prep(small_sheet(n=24, n_steps=200)), _prep(small_sheet(n=8, n_steps=200))
+    solver = runtime.SolverConfig(bricks_per_wave=bpw)
+    alone = runtime.Arena(pb.n_voxel, 1, solver)
+    s = alone.add(pb)
+    alone.step(pb.n_steps)
+    ua, _, la = alone.outputs(s)
+    mixed = runtime.Arena(pa.n_voxel + pb.n_voxel, 2, solver)
+    mixed.add(pa)
+    mixed.step(50, graph=True)
+    g0 = mixed.counts()["launch_bricks"]
+    sb = mixed.add(pb)
+    print(f"bricks {mixed.counts()['bricks']} launch bricks {g0} -> {mixed.counts()['launch_bricks']}")
+    while mixed.steps_done(sb) < pb.n_steps:
+        mixed.step(50, graph=True)
+    um, _, lm = mixed.outputs(sb)
+    assert np.array_equal(ua.cpu().numpy(), um.cpu().numpy())
+    assert np.array_equal(la.cpu().numpy(), lm.cpu().numpy())
