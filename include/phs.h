@@ -251,6 +251,17 @@ int phs_egm_normalize(float* egm, int64_t n_time, int32_t n_electrode, double ty
  * [n_electrode].  No allocation; stream-ordered. */
 int phs_egm_electrode_lat(const float* egm, int64_t n_time, int32_t n_electrode, int32_t* lat,
                           void* stream);
+/* Conduction velocity at scattered points (replaces the fit loop of
+ * utility/conduction_velocity.py:23-70): for each point i, the plane lat = a x + b y + c z + d
+ * fitted by least squares (minimum-norm, as np.linalg.lstsq) to the points j with
+ * (dx^2 + dy^2) + dz^2 <= radius^2 (i included); g = (a, b, c); velocity[i] = g / |g|^2 and
+ * magnitude[i] = 1 / |g| if |g|^2 > 1e-8, else NaN, also NaN with fewer than 4 points.
+ * mean (optional, may be NULL) = mean of the non-NaN magnitudes (NaN if none).
+ * xyz device double [n_points][3] (mm), lat device double [n_points]; velocity device double
+ * [n_points][3], magnitude device double [n_points], mean device double [1].  FP64; no
+ * allocation; stream-ordered. */
+int phs_conduction_velocity(const double* xyz, const double* lat, int32_t n_points, double radius,
+                            double* velocity, double* magnitude, double* mean, void* stream);
 
 #ifdef __cplusplus
 }

@@ -112,6 +112,7 @@ SIGNATURES = {
     "phs_egm_destroy": (C.c_int, [_P]),
     "phs_egm_normalize": (C.c_int, [_P, _I64, _I32, _D, _P]),
     "phs_egm_electrode_lat": (C.c_int, [_P, _I64, _I32, _P, _P]),
+    "phs_conduction_velocity": (C.c_int, [_P, _P, _I32, _D, _P, _P, _P, _P]),
 }
 
 _lib = None
