@@ -148,6 +148,16 @@ def cpu_baseline(work, u0, h0, step0, budget_s):
                 maxiter_hits=int(fails), seconds=dt)
 
 
+def fewest_stable_sweeps(runtime, preps, guess):
+    """(sweeps, lambda_max): the fewest Chebyshev sweeps whose inexact CN step grows no mode
+    (amplification <= 1 over the Gershgorin interval [1, lambda_max]) for the largest spectral
+    bound among `preps`, read from a probe arena."""
+    probe = runtime.Arena(sum(p.n_voxel for p in preps), len(preps), runtime.SolverConfig(sweeps=0))
+    lam = max(probe.bounds(probe.add(p, samples=False, max_beats=1))[1] for p in preps)
+    probe.close()
+    return runtime.stable_sweeps(lam, guess)[0], lam
+
+
 def run_ragged(args, torch, runtime):
     """64 seeded shells of uneven size (0.4 mm, 1000 ms, 0-3 scar blobs, CL 250-600 ms) packed
     into one arena; throughput over --steps graph-replayed steps; LAT of two of them against
@@ -163,7 +173,7 @@ def run_ragged(args, torch, runtime):
     ns = [p.n_voxel for p in preps]
     log(f"ragged: {len(preps)} sims, N {min(ns)}-{max(ns)}, total {sum(ns)}, setup "
         f"{time.perf_counter()-t0:.1f}s")
-    solver = runtime.SolverConfig(sweeps=args.sweeps, guess=args.guess)
+    solver = runtime.SolverConfig(sweeps=max(args.sweeps, 0), guess=args.guess)
     arena = runtime.Arena(sum(ns), len(preps), solver)
     slots = [arena.add(p, samples=False) for p in preps]
     G = args.graph_steps
@@ -221,7 +231,7 @@ def run_precision(args, torch, runtime):
     # automatic sweep count (stability + accuracy rule) without refinement, and with three
     # FP64-residual refinement passes of 12 sweeps (CN residual <= 1e-10)
     for refine, rsw in ((0, 8), (3, 12)):
-        cfg = runtime.SolverConfig(sweeps=args.sweeps, guess=args.guess, refine=refine,
+        cfg = runtime.SolverConfig(sweeps=max(args.sweeps, 0), guess=args.guess, refine=refine,
                                    refine_sweeps=rsw, diag=True)
         arena = runtime.Arena(N, 1, cfg)
         slot = arena.add(prep, samples=False)
@@ -276,8 +286,11 @@ def main():
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--sweeps", type=int, default=0,
-                    help="Chebyshev sweeps per CN solve; 0 = automatic (stability rule)")
+    ap.add_argument("--sweeps", type=int, default=-1,
+                    help="Chebyshev sweeps per CN solve; -1 = the fewest that keep the step "
+                         "stable for the workload's spectral bound (shell workload; the "
+                         "validation leg checks LAT <= 1 dt in the same run), 0 = the arena's "
+                         "automatic count (one more)")
     ap.add_argument("--guess", type=int, default=1)
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--qps", type=float, default=0.5)
@@ -311,7 +324,11 @@ def main():
     log(f"workload: N={N} steps={S_full} batch={args.batch} setup {time.perf_counter()-t_setup:.1f}s")
     K, W = args.steps, args.warmup
     G = args.graph_steps
-    solver = runtime.SolverConfig(sweeps=args.sweeps, guess=args.guess, brick=(4, 4, 4))
+    sweeps = args.sweeps
+    if sweeps < 0:
+        sweeps, lam = fewest_stable_sweeps(runtime, preps, args.guess)
+        log(f"sweeps: {sweeps} (fewest stable for lambda_max = {lam:.4f}, guess {args.guess})")
+    solver = runtime.SolverConfig(sweeps=sweeps, guess=args.guess, brick=(4, 4, 4))
     if args.calibrate_copy > 0:
         n = args.calibrate_copy * (1 << 20) // 4
         src = torch.rand(n, device="cuda")
@@ -457,6 +474,8 @@ def main():
                         f"0.4 mm voxels (N={N}), dt 0.05 ms, CL 500 ms, x{B} simulations packed "
                         f"(different seeded pacing sites)",
             "n_voxel": N, "batch": B, "steps_per_sim": S_full, "sweeps": sweeps_used,
+            "sweep_rule": ("fewest stable for the spectral bound" if args.sweeps < 0 else
+                           "automatic (fewest stable + 1)" if args.sweeps == 0 else "fixed"),
             "guess": args.guess, "brick": [4, 4, 4], "graph_steps": G,
         },
         "speedup_vs_plain_fp64": (value / res["plain"]["voxel_steps_per_s"]
