@@ -239,3 +239,40 @@ def test_graph_recaptured_when_bricks_added_within_grid(cuda, bpw):
     um, _, lm = mixed.outputs(sb)
     assert np.array_equal(ua.cpu().numpy(), um.cpu().numpy())
     assert np.array_equal(la.cpu().numpy(), lm.cpu().numpy())
+
+
+def _hetero_c(w, seed=11):
+    """Per-voxel conductivity (P_2d column 20, as the reference builds it from c_voxel) varying
+    smoothly in [0.6, 1.4] x the configured value, with one region at 0 (scar by conductivity)."""
+    rng = np.random.default_rng(seed)
+    P = w["P_2d"].copy()
+    v = w["geometry"]["voxel"]
+    k = rng.normal(size=3)
+    f = 0.6 + 0.8 * (0.5 + 0.5 * np.sin(v @ k / 3.0))
+    P[:, 20] = P[:, 20] * f
+    P[np.linalg.norm(v - v[len(v) // 3], axis=1) < 1.2, 20] = 0.0
+    w["P_2d"] = P
+    return w
+
+
+def test_plain_fp64_matches_oracle_heterogeneous_c(cuda):
+    w = _hetero_c(small_shell(t_final=40.0))
+    ref = run_oracle(w, max_beats=7)
+    prep, sim = _plain(w)
+    assert prep.c_voxel is not None
+    err = np.max(np.abs(sim.su.cpu().numpy() - ref["u"])) / np.max(np.abs(ref["u"]))
+    print("rel Vm err", err)
+    assert err <= 1e-9
+    assert np.array_equal(sim.lat.cpu().numpy().astype(np.int64), ref["lat"][:7])
+
+
+def test_fp32_arena_heterogeneous_c_lat_within_one_step(cuda):
+    from pyheartsim_amd import runtime
+    w = _hetero_c(small_shell(t_final=40.0))
+    ref = run_oracle(w, max_beats=7)
+    prep = _prep(w)
+    arena = runtime.Arena(prep.n_voxel, 1, runtime.SolverConfig())
+    slot = arena.add(prep)
+    arena.step(prep.n_steps)
+    _, _, lat = arena.outputs(slot)
+    assert _lat_ok(lat.cpu().numpy(), ref["lat"][:7], 1) <= 1
