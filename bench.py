@@ -42,23 +42,30 @@ BF16_PEAK_TFLOPS = 2500.0
 
 
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "arena_sweep_pmc_shell_ms_x16.md")
+PROFILE_TRAFFIC_EGM = os.path.join(ROOT, "profiles", "egm_shell_e64_t2000.md")
 
 
-def profiled_traffic(cnt):
-    """HBM bytes (read + write) of one middle-sweep launch, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (calibrated on a device copy), when that summary was
-    measured on an arena of the same voxels and bricks; else None."""
+def _profile_record(path, match, key):
+    """`key` of the `traffic:` JSON line of a committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (calibrated on a device copy) whose fields equal `match`; else None."""
     try:
-        with open(PROFILE_TRAFFIC) as f:
+        with open(path) as f:
             for line in f:
                 if line.startswith("traffic:"):
                     rec = json.loads(line[len("traffic:"):])
-                    if (rec["voxels"] == cnt["voxels"] and rec["bricks"] == cnt["bricks"]
-                            and rec["record_bytes"] == BRICK_RECORD_BYTES):
-                        return rec["sweep_bytes_per_launch"]
+                    if all(rec.get(k) == v for k, v in match.items()):
+                        return rec[key]
     except (OSError, ValueError, KeyError):
         pass
     return None
+
+
+def profiled_traffic(cnt):
+    """HBM bytes (read + write) of one middle-sweep launch, when the committed summary was
+    measured on an arena of the same voxels and bricks."""
+    return _profile_record(PROFILE_TRAFFIC, {"voxels": cnt["voxels"], "bricks": cnt["bricks"],
+                                             "record_bytes": BRICK_RECORD_BYTES},
+                           "sweep_bytes_per_launch")
 
 
 def log(*a):
@@ -204,6 +211,64 @@ def run_ragged(args, torch, runtime):
                 sweep_gbs=bytes_sweep / (prof["sweep"] * 1e-3) / 1e9, validate=val)
 
 
+def run_egm(args, torch, runtime):
+    """The electrogram of the 0.4 mm shell alone: its 64 electrodes x 2000 samples over the
+    199 446 voxels (BF16 hi/lo MFMA contraction with the precomputed lead field), synthetic
+    plane-wave samples, --steps timed launches after --warmup; rel-L2 against the FP32 MFMA
+    form on the same samples.  One launch reads the samples (4 T N bytes) and the BF16 hi/lo
+    lead field (4 E N) and writes the T x E result: at E = 64 the useful intensity (2 E T N
+    flop over those bytes) is ~32 flop/B, so HBM bounds it."""
+    from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
+    g, work, preps = build_workload(range(1))
+    N = preps[0].n_voxel
+    el = work["electrodes"]
+    E, T = el.shape[0], 2000
+    lf = LeadField(el, g["voxel"], None, work["hm"]["c_voxel"], g["Delta"], g["neighbor_id_2d"])
+    x = torch.from_numpy(g["voxel"][:, 0].astype(np.float32)).cuda()
+    tt = torch.arange(T, device="cuda", dtype=torch.float32)[:, None]
+    samples = torch.sigmoid(tt * 0.03 - (x[None, :] + 25.0) * 0.3)   # a plane wave in x
+    e_ref = lf.run(samples, precision="fp32", normalize=False)
+    out = torch.empty((T, E), dtype=torch.float32, device="cuda")
+    for _ in range(max(args.warmup, 1)):
+        lf.run(samples, precision="bf16x3", normalize=False, out=out)
+    K = max(args.steps, 1)
+    st = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(st)
+    for _ in range(K):
+        lf.run(samples, precision="bf16x3", normalize=False, out=out)
+    ev1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev0.elapsed_time(ev1) / K
+    rel = float(((out - e_ref).norm() / e_ref.norm()).item())
+    useful = 2.0 * E * T * N
+    nbytes = 4.0 * T * N + 4.0 * E * N + 4.0 * T * E
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    log(f"egm: E={E} T={T} N={N} {ms:.4f} ms/launch, rel_l2 {rel:.3e}, {gbs:.0f} GB/s")
+    return {
+        "metric": "electrogram useful TFLOP/s (2 E T N per launch, BF16 hi/lo MFMA)",
+        "value": useful / (ms * 1e-3) / 1e12,
+        "unit": "TFLOP/s",
+        "n_gpus": 1, "steps": K, "warmup": max(args.warmup, 1), "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
+        "data": "synthetic",
+        "config": {"workload": f"egm: 0.4 mm shell lead field, E={E} electrodes, T={T} samples, "
+                               f"N={N} voxels", "electrodes": E, "samples": T, "n_voxel": N},
+        "roofline": {"bound": "hbm", "kernel": "egm_gemm_lds + egm_reduce (one electrogram)",
+                     "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": nbytes,
+                     "traffic": _profile_record(PROFILE_TRAFFIC_EGM,
+                                                {"n_voxel": N, "electrodes": E, "samples": T},
+                                                "bytes_per_launch")},
+        "mfma_issued_tflops": 3.0 * useful / (ms * 1e-3) / 1e12,
+        "rel_l2_vs_fp32_mfma": rel,
+        "wall_s": wall,
+    }
+
+
 def run_precision(args, torch, runtime):
     """The 0.2 mm shell with 3 scar blobs: FP32 sweeps with FP64-residual refinement, the
     achieved CN residual, LAT against the plain FP64 form over a window, and the 256-electrode
@@ -279,7 +344,8 @@ def run_precision(args, torch, runtime):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--workload", choices=["shell_ms", "ragged", "precision"], default="shell_ms")
+    ap.add_argument("--workload", choices=["shell_ms", "ragged", "precision", "egm"],
+                    default="shell_ms")
     ap.add_argument("--ragged-sims", type=int, default=64)
     ap.add_argument("--precision-ms", type=float, default=300.0)
     ap.add_argument("--gpus", type=int, default=1)
@@ -312,8 +378,17 @@ def main():
     from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.rand(n, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        del src, dst
     if args.workload != "shell_ms":
-        fn = run_ragged if args.workload == "ragged" else run_precision
+        fn = {"ragged": run_ragged, "precision": run_precision, "egm": run_egm}[args.workload]
         print(json.dumps(fn(args, torch, runtime)), flush=True)
         return
     t_setup = time.perf_counter()
@@ -329,15 +404,6 @@ def main():
         sweeps, lam = fewest_stable_sweeps(runtime, preps, args.guess)
         log(f"sweeps: {sweeps} (fewest stable for lambda_max = {lam:.4f}, guess {args.guess})")
     solver = runtime.SolverConfig(sweeps=sweeps, guess=args.guess, brick=(4, 4, 4))
-    if args.calibrate_copy > 0:
-        n = args.calibrate_copy * (1 << 20) // 4
-        src = torch.rand(n, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
-        del src, dst
     res = {}
 
     ks = (K // G) * G
