@@ -466,9 +466,13 @@ def main():
             ms = ev0.elapsed_time(ev1) / reps
             E = el.shape[0]
             useful = 2.0 * E * T * N
+            egm_bytes = 4.0 * T * N + 4.0 * E * N + 4.0 * T * E
             res["egm"] = dict(rel_l2=rel, ms=ms, electrodes=E, samples=T,
                               useful_tflops=useful / ms / 1e9,
                               mfma_issued_tflops=3 * useful / ms / 1e9,
+                              mfma_issued_frac=3 * useful / ms / 1e9 / BF16_PEAK_TFLOPS,
+                              hbm_gbs=egm_bytes / ms / 1e6,
+                              hbm_frac=egm_bytes / ms / 1e6 / HBM_PEAK_GBS,
                               reference_equivalent_tflops=2.0 * E * T * 3 * N / ms / 1e9)
             log(f"egm: {res['egm']}")
             del plain, single
