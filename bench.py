@@ -354,10 +354,12 @@ def main():
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--sweeps", type=int, default=0,
                     help="Chebyshev sweeps per CN solve; 0 = the arena's automatic count (the "
-                         "fewest that keep the step stable for the spectral bound, plus one), "
+                         "fewest that keep the step stable for the spectral bound, with margin), "
                          "-1 = the fewest stable (shell workload: 5 at lambda_max 8.5; its "
                          "validation meets LAT <= 1 dt but not the 1e-3 electrogram bar)")
-    ap.add_argument("--guess", type=int, default=1)
+    ap.add_argument("--guess", type=int, default=2,
+                    help="warm start of the CN correction: 0 none, 1 previous step, 2 linear "
+                         "extrapolation of the previous two")
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--qps", type=float, default=0.5)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
@@ -545,7 +547,7 @@ def main():
                         f"(different seeded pacing sites)",
             "n_voxel": N, "batch": B, "steps_per_sim": S_full, "sweeps": sweeps_used,
             "sweep_rule": ("fewest stable for the spectral bound" if args.sweeps < 0 else
-                           "automatic (fewest stable + 1)" if args.sweeps == 0 else "fixed"),
+                           "automatic (stable with margin)" if args.sweeps == 0 else "fixed"),
             "guess": args.guess, "brick": [4, 4, 4], "graph_steps": G,
         },
         "speedup_vs_plain_fp64": (value / res["plain"]["voxel_steps_per_s"]

@@ -63,8 +63,10 @@ static double max_root_modulus(int n, const double* c) {
   return r;
 }
 
-double step_amplification(double lmax, int guess, int sweeps, int refine, int refine_sweeps) {
+double step_amplification(double lmax, int guess, int sweeps, int refine, int refine_sweeps,
+                          double lam_lo) {
   if (!(lmax > 1.0)) return 1.0;
+  if (!(lam_lo < lmax)) return 0.0;
   const int kmax = std::max(sweeps, refine_sweeps);
   std::vector<ChebCoef> cf(std::max(kmax, 1));
   chebyshev_coefficients(1.0, lmax, kmax, cf.data());
@@ -81,7 +83,7 @@ double step_amplification(double lmax, int guess, int sweeps, int refine, int re
   const int n = 4096;
   double worst = 0.0;
   for (int i = 0; i <= n; ++i) {
-    const double lam = 1.0 + (lmax - 1.0) * i / n;
+    const double lam = lam_lo + (lmax - lam_lo) * i / n;
     double P = poly(lam, sweeps);
     for (int r = 0; r < refine; ++r) P *= poly(lam, refine_sweeps);
     const double a = 1.0 + (1.0 - P) * 2.0 * (1.0 - lam) / lam;
@@ -119,8 +121,12 @@ int auto_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax)
   const int k0 = stable_sweeps(lmax, guess, refine, refine_sweeps, kmax);
   if (k0 < 0) return -1;
   const double tail = std::pow(cheb_bound(lmax, refine_sweeps), refine);
+  const double stiff_max = 1.0 - 1.0 / kStiffSteps;
   for (int k = k0; k <= kmax; ++k)
-    if (lmax * cheb_bound(lmax, k) * tail <= 0.25) return k;
+    if (lmax * cheb_bound(lmax, k) * tail <= 0.25 &&
+        (lmax <= 2.0 ||
+         step_amplification(lmax, guess, k, refine, refine_sweeps, 2.0) <= stiff_max))
+      return k;
   return kmax;
 }
 

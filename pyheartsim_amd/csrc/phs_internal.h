@@ -31,14 +31,21 @@ int chebyshev_sweeps(double lmin, double lmax, double tol);
 //   guess 1: (u, d) -> companion of mu^2 - (a + P) mu + P
 //   guess 2: (u, d, d_prev) -> companion of mu^3 - (a + 2P) mu^2 + 3P mu - P
 // (exact solve: P = 0, u' = (2/lam - 1) u, the CN factor).  Returns the largest root
-// modulus over a 4097-point grid of [1, lmax]; > 1 means the step grows that mode.
-double step_amplification(double lmax, int guess, int sweeps, int refine, int refine_sweeps);
+// modulus over a 4097-point grid of [lam_lo, lmax] (default the whole spectrum [1, lmax]);
+// > 1 means the step grows that mode.
+double step_amplification(double lmax, int guess, int sweeps, int refine, int refine_sweeps,
+                          double lam_lo = 1.0);
 // Fewest sweeps (2..kmax) whose step amplification is <= 1, or -1.
 int stable_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax);
-// Automatic count: the fewest stable sweeps m for which the solve error at the stiffest
-// mode stays under a quarter of CN's own damping there, lmax * max|P| <= 1/4 with the
-// Chebyshev bound max|p_m| = 2 q^m / (1 + q^2m), q = (sqrt(lmax) - 1) / (sqrt(lmax) + 1)
+// Automatic count: the fewest stable sweeps m for which
+//  (1) the solve error at the stiffest mode stays under a quarter of CN's own damping there,
+//      lmax * max|P| <= 1/4 with the Chebyshev bound max|p_m| = 2 q^m / (1 + q^2m),
+//      q = (sqrt(lmax) - 1) / (sqrt(lmax) + 1), and
+//  (2) every stiff mode (lam >= 2, the modes CN damps with a change of sign) shrinks by a
+//      factor e within kStiffSteps steps: amplification over [2, lmax] <= 1 - 1/kStiffSteps.
+//      The smooth modes (lam < 2) keep CN's physical factor close to 1.
 // (clamped to kmax; -1 when no count is stable).
+constexpr double kStiffSteps = 40.0;
 int auto_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax);
 
 }  // namespace phs

@@ -33,7 +33,7 @@ from .. import runtime
 
 
 def compute(n_voxel, P_2d, geometry_data, simulation_parameters, arrhythmia_parameters, *,
-            precision: str = "fp32", sweeps: int = 0, guess: int = 1, refine: int = 0,
+            precision: str = "fp32", sweeps: int = 0, guess: int = 2, refine: int = 0,
             refine_sweeps: int = 8, brick=(4, 4, 4), plain_tol: float = 1e-15,
             return_lat: bool = False, device_out: bool = False):
     prep = runtime.prepare(n_voxel, P_2d, geometry_data, simulation_parameters,

@@ -10,7 +10,11 @@ Pins:
     grows without bound two sweep counts below the stability limit and stays bounded at
     the limit (a plain-matrix restatement, no library code);
   * the counts measured on the 0.2 mm shell (lambda_max = 31): 14 sweeps is the stability
-    boundary with warm start, 16 the automatic count; 6 for the 0.4 mm shell (8.5).
+    boundary with warm start, 16 the automatic count; 6 for the 0.4 mm shell (8.5);
+  * the automatic count is the fewest stable sweeps that meet the accuracy bound and make
+    every stiff mode (lam >= 2) shrink by e within 40 steps, recomputed from the mode
+    matrices; with extrapolated warm start (guess 2) at lambda_max 31 that is 17 (16, the
+    stability boundary, gave activation mismatches on the 0.2 mm shell; 17 met LAT <= 1).
 """
 import numpy as np
 import pytest
@@ -85,6 +89,8 @@ def test_exact_solve_reduces_to_cn_factor():
 def test_counts_measured_on_shells():
     assert runtime.stable_sweeps(31.0, 1) == (14, 16)
     assert runtime.stable_sweeps(8.5, 1) == (5, 6)
+    assert runtime.stable_sweeps(31.0, 2) == (16, 17)
+    assert runtime.stable_sweeps(8.5, 2) == (6, 6)
     assert runtime.step_amplification(31.0, 13, 1) > 1.005
     assert runtime.step_amplification(31.0, 14, 1) <= 1.0 + 1e-9
 
@@ -119,3 +125,25 @@ def test_chain_diverges_below_limit_and_not_at_it():
     start = _chain_run(64, lmax, steps=1)
     assert _chain_run(stable, lmax) <= 10.0 * start
     assert _chain_run(stable - 2, lmax) > 1e3 * start
+
+
+def _stiff_radius(lmax, m, guess):
+    return max(mode_radius(l, lmax, m, guess) for l in np.linspace(2.0, lmax, 4097))
+
+
+def _accurate(lmax, m):
+    q = (np.sqrt(lmax) - 1.0) / (np.sqrt(lmax) + 1.0)
+    return lmax * 2.0 * q ** m / (1.0 + q ** (2 * m)) <= 0.25
+
+
+@pytest.mark.parametrize("lmax,guess", [(8.5, 1), (8.5, 2), (16.0, 2), (31.0, 1), (31.0, 2),
+                                        (60.0, 1)])
+def test_automatic_count_rule(lmax, guess):
+    stable, auto = runtime.stable_sweeps(lmax, guess)
+
+    def ok(m):
+        return (m >= stable and _accurate(lmax, m)
+                and _stiff_radius(lmax, m, guess) <= 1.0 - 1.0 / 40.0)
+
+    assert ok(auto)
+    assert not any(ok(m) for m in range(stable, auto))
