@@ -108,7 +108,9 @@ int phs_device_count(int32_t* count);
  * the stiffest modes by *amplification per step (> 1 = unstable; the exact solve gives the
  * CN factor, <= 1).  phs_stable_sweeps gives the fewest sweeps with amplification <= 1
  * (*stable) and the arena's automatic count (*automatic, may be NULL): the fewest stable
- * sweeps whose error bound at the stiffest mode, lambda_max * max|P|, is <= 1/4.
+ * sweeps whose error bound at the stiffest mode, lambda_max * max|P|, is <= 1/4 and under
+ * which every stiff mode (lambda >= 2) shrinks by e within 40 steps (amplification over
+ * [2, lambda_max] <= 1 - 1/40).
  * No counterpart in the reference, whose BiCGSTAB solve runs to a tolerance. */
 int phs_stable_sweeps(double lambda_max, int32_t guess, int32_t refine, int32_t refine_sweeps,
                       int32_t* stable, int32_t* automatic);

@@ -227,7 +227,7 @@ class PlainSim:
 
 
 # --------------------------------------------------------------------- bricked FP32 arena
-def stable_sweeps(lambda_max: float, guess: int = 1, refine: int = 0, refine_sweeps: int = 8):
+def stable_sweeps(lambda_max: float, guess: int = 2, refine: int = 0, refine_sweeps: int = 8):
     """(fewest Chebyshev sweeps whose inexact CN step grows no mode, the arena's automatic
     count) for spectral bound lambda_max (phs_stable_sweeps)."""
     lib = _lib.load()
@@ -237,7 +237,7 @@ def stable_sweeps(lambda_max: float, guess: int = 1, refine: int = 0, refine_swe
     return s.value, a.value
 
 
-def step_amplification(lambda_max: float, sweeps: int, guess: int = 1, refine: int = 0,
+def step_amplification(lambda_max: float, sweeps: int, guess: int = 2, refine: int = 0,
                        refine_sweeps: int = 8) -> float:
     """Largest per-step growth factor of any mode of the inexact CN step (phs_step_amplification)."""
     lib = _lib.load()
