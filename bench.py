@@ -168,8 +168,10 @@ def fewest_stable_sweeps(runtime, preps, guess):
 def run_ragged(args, torch, runtime):
     """64 seeded shells of uneven size (0.4 mm, 1000 ms, 0-3 scar blobs, CL 250-600 ms) packed
     into one arena; throughput over --steps graph-replayed steps; LAT of two of them against
-    the plain FP64 form over the full 1000 ms."""
-    from pyheartsim_amd import configs
+    the plain FP64 form over the full 1000 ms.  The brick shape and bricks per wavefront come
+    from the autotuner (pyheartsim_amd/autotune.py), run on every 8th shell: the two shapes of
+    best brick fill, one or two bricks per wavefront, timed on the device."""
+    from pyheartsim_amd import autotune, configs
     t0 = time.perf_counter()
     ws = configs.ragged(args.ragged_sims)
     preps = []
@@ -180,7 +182,16 @@ def run_ragged(args, torch, runtime):
     ns = [p.n_voxel for p in preps]
     log(f"ragged: {len(preps)} sims, N {min(ns)}-{max(ns)}, total {sum(ns)}, setup "
         f"{time.perf_counter()-t0:.1f}s")
-    solver = runtime.SolverConfig(sweeps=max(args.sweeps, 0), guess=args.guess)
+    base = runtime.SolverConfig(sweeps=max(args.sweeps, 0), guess=args.guess)
+    t0 = time.perf_counter()
+    solver, tune_table = autotune.tune(preps[::8], base, steps=200, warmup=50,
+                                       graph_steps=args.graph_steps)
+    tune = dict(brick=list(solver.brick), bricks_per_wave=solver.bricks_per_wave,
+                seconds=time.perf_counter() - t0,
+                candidates=[dict(brick=r["brick"], bricks_per_wave=r["bricks_per_wave"],
+                                 fill=r["fill"], voxel_steps_per_s=r["voxel_steps_per_s"])
+                            for r in tune_table])
+    log(f"ragged autotune: {tune}")
     arena = runtime.Arena(sum(ns), len(preps), solver)
     slots = [arena.add(p, samples=False) for p in preps]
     G = args.graph_steps
@@ -206,7 +217,7 @@ def run_ragged(args, torch, runtime):
     log(f"ragged validation: {val}")
     bytes_sweep = cnt["voxels"] * 16 + cnt["vf_voxels"] + cnt["bricks"] * BRICK_RECORD_BYTES
     return dict(workload="ragged", sims=len(preps), voxels=sum(ns), voxel_steps_per_s=value,
-                sweeps=arena.sweeps(),
+                sweeps=arena.sweeps(), autotune=tune,
                 ms_per_step=1e3 * t / ks, steps=ks, kernels_ms=prof,
                 sweep_gbs=bytes_sweep / (prof["sweep"] * 1e-3) / 1e9, validate=val)
 
