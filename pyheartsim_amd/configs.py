@@ -116,6 +116,9 @@ def sheet_ap(n_steps: int = 2000, nx: int = 128, ny: int = 128, n_probes: int = 
                 n_steps=n_steps)
 
 
+AP_SHELL_STIMULUS = 3.0  # model-time units of the Aliev-Panfilov shells' stimulus
+
+
 def shell_ms(delta: float = 0.4, t_final: float = 2000.0, seed: int = 0, n_electrodes: int = 64,
              cycle_length: float = 500.0, dt: float = 0.05, tau_close: float = 150.0,
              semi_axes=(25.0, 20.0, 18.0), wall: float = 3.0, n_scar: int = 0,
@@ -135,6 +138,11 @@ def shell_ms(delta: float = 0.4, t_final: float = 2000.0, seed: int = 0, n_elect
         ms=dict(tau_close=tau_close))
     if zero_c_in_scar:
         hm["c_voxel"] = np.where(node_flag == 4, 0.0, hm["c_voxel"])
+    if heart_model_flag == 1:
+        # Aliev-Panfilov: the reference's 5 / time_scale stimulus (pacing.py:127-132) on the
+        # 2 mm pacing site is drained by diffusion (c = 9.6) before the upstroke and never
+        # excites the shell; 3 model-time units (18 ms) do, as the 2D sheet's 2 units do
+        sim["pacing_duration_override"] = AP_SHELL_STIMULUS
     electrodes = geo.fibonacci_electrodes(n_electrodes, 15.0)
     return dict(name="shell_ms", geometry=g, sim=sim, arr=arr, hm=hm, electrodes=electrodes)
 

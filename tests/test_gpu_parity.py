@@ -276,3 +276,23 @@ def test_fp32_arena_heterogeneous_c_lat_within_one_step(cuda):
     arena.step(prep.n_steps)
     _, _, lat = arena.outputs(slot)
     assert _lat_ok(lat.cpu().numpy(), ref["lat"][:7], 1) <= 1
+
+
+def test_aliev_panfilov_shell_against_oracle(cuda):
+    """The 3D Aliev-Panfilov path (time scaling by 6, AP reaction): plain FP64 form within
+    1e-9 relative Vm of the oracle with identical LAT, the bricked FP32 form within 1 step."""
+    from pyheartsim_amd import runtime
+    w = small_shell(t_final=80.0, heart_model_flag=1)
+    ref = run_oracle(w, max_beats=7)
+    assert (ref["lat"][0] >= 0).mean() > 0.9     # the wave crosses the shell
+    prep, sim = _plain(w)
+    assert prep.model == 1
+    err = np.max(np.abs(sim.su.cpu().numpy() - ref["u"])) / np.max(np.abs(ref["u"]))
+    print("AP rel Vm err", err)
+    assert err <= 1e-9
+    assert np.array_equal(sim.lat.cpu().numpy().astype(np.int64), ref["lat"][:7])
+    arena = runtime.Arena(prep.n_voxel, 1, runtime.SolverConfig())
+    slot = arena.add(prep)
+    arena.step(prep.n_steps)
+    _, _, lat = arena.outputs(slot)
+    assert _lat_ok(lat.cpu().numpy(), ref["lat"][:7], 1) <= 1
