@@ -119,14 +119,15 @@ def select_sweeps(prep: runtime.Prepared, ref_lat, base: runtime.SolverConfig,
 
 
 if __name__ == "__main__":
+    # python -m pyheartsim_amd.autotune [n]: tune over n copies (default 16) of the 0.4 mm
+    # shell with different pacing sites, every shape and both workgroup mappings
     import json
     import sys
 
     from . import configs
-    g = None
-    preps = []
     from . import geometry as geo
     g = geo.shell((25.0, 20.0, 18.0), 3.0, 0.4, 4, 6.0, 0)
+    preps = []
     for s in range(int(sys.argv[1]) if len(sys.argv) > 1 else 16):
         w = configs.shell_ms(seed=s, geometry=g)
         P = configs.build_P_2d(g, w["sim"], w["hm"])
