@@ -253,6 +253,28 @@ int phs_egm_normalize(float* egm, int64_t n_time, int32_t n_electrode, double ty
  * [n_electrode].  No allocation; stream-ordered. */
 int phs_egm_electrode_lat(const float* egm, int64_t n_time, int32_t n_electrode, int32_t* lat,
                           void* stream);
+/* Activation markers and local activation times of sampled action potentials (replaces
+ * legacy/calculate_local_activation_time.py:18-66 execute_on_action_potential with the
+ * markers of :67-136 find_action_potential_marker).  ap: device float, sample t of signal i at
+ * ap[t * stride + i] (stride = n_signal for a row-major [n_time][n_signal] array).  Markers of a signal: for each pair of the sorted unique boundary points
+ * of its runs above `threshold`, the first index of the largest forward difference, kept when
+ * that difference is > 0 (or NaN).  lat[i] = the first marker in [t_start, t_start +
+ * round(CL * cycle_length_fraction)] (CL = mean cycle length over all signals, 180 if none),
+ * then lat - min(lat) + 1; NaN where there is none.  stats (device double[5], may be NULL):
+ * CL, window start, window end, total markers, total cycle lengths.  any_above (device uint8
+ * [n_signal], may be NULL): whether the signal exceeds the threshold.  workspace: device
+ * memory of phs_ap_lat_workspace bytes; it keeps the offsets that phs_ap_markers uses.  No
+ * allocation; stream-ordered. */
+int phs_ap_lat_workspace(int64_t n_signal, int64_t* bytes);
+int phs_ap_lat(const float* ap, int64_t n_time, int64_t n_signal, int64_t stride, double threshold,
+               double t_start, double cycle_length_fraction, void* workspace, double* lat,
+               double* stats, uint8_t* any_above, void* stream);
+/* After phs_ap_lat on the same workspace and samples: every signal's markers (device int32
+ * [total markers], signal-major, in time order) and cycle lengths (device double [total cycle
+ * lengths], the differences of consecutive markers of each signal), either may be NULL. */
+int phs_ap_markers(const float* ap, int64_t n_time, int64_t n_signal, int64_t stride,
+                   double threshold, void* workspace, int32_t* markers, double* cycle_lengths,
+                   void* stream);
 /* Conduction velocity at scattered points (replaces the fit loop of
  * utility/conduction_velocity.py:23-70): for each point i, the plane lat = a x + b y + c z + d
  * fitted by least squares (minimum-norm, as np.linalg.lstsq) to the points j with

@@ -113,6 +113,9 @@ SIGNATURES = {
     "phs_egm_normalize": (C.c_int, [_P, _I64, _I32, _D, _P]),
     "phs_egm_electrode_lat": (C.c_int, [_P, _I64, _I32, _P, _P]),
     "phs_conduction_velocity": (C.c_int, [_P, _P, _I32, _D, _P, _P, _P, _P]),
+    "phs_ap_lat_workspace": (C.c_int, [_I64, _PI64]),
+    "phs_ap_lat": (C.c_int, [_P, _I64, _I64, _I64, _D, _D, _D, _P, _P, _P, _P, _P]),
+    "phs_ap_markers": (C.c_int, [_P, _I64, _I64, _I64, _D, _P, _P, _P, _P]),
 }
 
 _lib = None
