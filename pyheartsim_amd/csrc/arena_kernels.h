@@ -85,20 +85,15 @@ struct ArenaDev {
   float* USTAR;
   uint8_t* VF;
   const float* ALPHA;
-  float* D1;  // CN correction of the previous step (warm start)
-  float* D2;  // correction of the step before (linear extrapolation)
+  float* D1;  // CN correction of the previous step (the warm start with guess 1)
+  float* D2;  // guess 2: the next step's warm start 2 D1 - (correction of the step before),
+              // formed in FP32 when D1 is written
   int32_t n_bricks;
   int32_t xcd;
   int32_t diag;
 };
 
 enum { MODE_MID = 0, MODE_FINAL_X = 1, MODE_DACC_SET = 2, MODE_DACC_ADD = 3, MODE_DACC_ADD_X = 4 };
-
-// Warm start of the CN correction d (x = u* + d): 0, the previous step's correction, or
-// the linear extrapolation 2 d_{n-1} - d_{n-2}.
-__device__ __forceinline__ float guess_of(int guess, float d1, float d2) {
-  return guess == 0 ? 0.0f : (guess == 1 ? d1 : 2.0f * d1 - d2);
-}
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
@@ -111,6 +106,21 @@ __device__ __forceinline__ T ld(const T* base, int i) {
 template <typename T>
 __device__ __forceinline__ void stv(T* base, int i, T v) {
   *(T*)((char*)base + (uint32_t)i * (uint32_t)sizeof(T)) = v;
+}
+
+// Warm start of the CN correction d (x = u* + d): 0, the previous step's correction d_{n-1}
+// (D1), or the linear extrapolation 2 d_{n-1} - d_{n-2}, kept ready in D2 so that the
+// right-hand side gathers one array for it.
+template <int GUESS>
+__device__ __forceinline__ const float* guess_array(const ArenaDev& A) {
+  return GUESS == 2 ? A.D2 : A.D1;
+}
+
+// Stores the step's total correction dt (own index i; o1 = the previous one, read from D1).
+template <int GUESS>
+__device__ __forceinline__ void keep_correction(const ArenaDev& A, int i, float o1, float dt) {
+  if (GUESS == 2) stv(A.D2, i, 2.0f * dt - o1);
+  stv(A.D1, i, dt);
 }
 
 __device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
@@ -396,8 +406,8 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
   Stencil<BX, BY, BZ> st[NB];
   SlotView sv[NB];
   bool on[NB];
-  float un[NB], d1[NB], d2[NB];
-  float gu[NB][6], g1[NB][6], g2[NB][6];
+  float un[NB], gn0[NB];
+  float gu[NB][6], gg[NB][6];
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int b = b0 + j;
@@ -413,12 +423,9 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
       un[j] = ld(A.U, st[j].idx);
       st[j].gather(A.U, gu[j]);
       if (GUESS >= 1) {
-        d1[j] = ld(A.D1, st[j].idx);
-        st[j].gather((const float*)A.D1, g1[j]);
-      }
-      if (GUESS == 2) {
-        d2[j] = ld(A.D2, st[j].idx);
-        st[j].gather((const float*)A.D2, g2[j]);
+        const float* G = guess_array<GUESS>(A);
+        gn0[j] = ld(G, st[j].idx);
+        st[j].gather(G, gg[j]);
       }
     }
   }
@@ -429,11 +436,8 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
     const double s = st[j].template sum<double>(uv, gu[j], 1.0f, lane);
     double sg = 0.0, gn = 0.0;
     if (GUESS >= 1) {
-      float q[6];
-#pragma unroll
-      for (int d = 0; d < 6; ++d) q[d] = guess_of(GUESS, g1[j][d], GUESS == 2 ? g2[j][d] : 0.0f);
-      const float gv = st[j].valid ? guess_of(GUESS, d1[j], GUESS == 2 ? d2[j] : 0.0f) : 0.0f;
-      sg = st[j].template sum<double>(gv, q, 1.0f, lane);
+      const float gv = st[j].valid ? gn0[j] : 0.0f;
+      sg = st[j].template sum<double>(gv, gg[j], 1.0f, lane);
       gn = gv;
     }
     if (!st[j].valid || !sv[j].live) continue;
@@ -476,8 +480,9 @@ __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotRec& S
 
 // One Chebyshev sweep d_{k+1} = a_k d_k + b_k d_{k-1} + g_k (r - A d_k), A d = d - alpha sum.
 // FIRST (k = 1): d_k = r / theta, read from R0.  prev_kind 0: d_{k-1} = 0; 1: r / theta;
-// 2: prev[].  GUESS: warm start of the correction (see guess_of); the final modes add it back
-// and keep the step's total correction in D1 (and the previous one in D2) for the next step.
+// 2: prev[].  GUESS: warm start of the correction (see guess_array); the final modes add it
+// back and keep the step's total correction in D1 (and, with guess 2, the next step's
+// extrapolated warm start in D2).
 template <int BX, int BY, int BZ, int MODE, int NB, int GUESS, bool FIRST>
 __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, const float* cur,
                                                   const float* prev, int prev_kind, float* out) {
@@ -531,9 +536,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       float dt = dnew;
       if (GUESS >= 1) {
         const float o1 = ld(A.D1, i);
-        dt += guess_of(GUESS, o1, GUESS == 2 ? ld(A.D2, i) : 0.0f);
-        if (GUESS == 2) stv(A.D2, i, o1);
-        stv(A.D1, i, dt);
+        dt += GUESS == 2 ? ld(A.D2, i) : o1;
+        keep_correction<GUESS>(A, i, o1, dt);
       }
       if (A.diag) {
         A.USTAR[i] = us;
@@ -542,7 +546,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       finish_voxel(A, S, i, us + dt);
     } else if (MODE == MODE_DACC_SET) {
       double base = 0.0;
-      if (GUESS >= 1) base = guess_of(GUESS, A.D1[i], GUESS == 2 ? A.D2[i] : 0.0f);
+      if (GUESS >= 1) base = guess_array<GUESS>(A)[i];
       A.DACC[i] = base + (double)dnew;
     } else if (MODE == MODE_DACC_ADD) {
       A.DACC[i] += (double)dnew;
@@ -550,10 +554,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       const double acc = A.DACC[i] + (double)dnew;
       const float us = A.U[i];
       A.DACC[i] = acc;
-      if (GUESS >= 1) {
-        if (GUESS == 2) A.D2[i] = A.D1[i];
-        A.D1[i] = (float)acc;
-      }
+      if (GUESS >= 1) keep_correction<GUESS>(A, i, A.D1[i], (float)acc);
       if (A.diag) A.USTAR[i] = us;
       finish_voxel(A, S, i, (float)((double)us + acc));
     }
