@@ -133,15 +133,16 @@ __device__ __forceinline__ int mbcnt64(uint64_t m) {
 
 // First brick of this wavefront.  With `xcd`, the round-robin dispatch of workgroups over
 // the 8 XCDs is undone so that each XCD works on a contiguous run of bricks and the halo
-// gathers hit that XCD's L2 (the grid is then a multiple of 8).
+// gathers hit that XCD's L2 (the grid is then a multiple of 8).  The remap is computed
+// either way and selected, so that the kernel-argument loads issue together, not behind a
+// branch.
 template <int NB>
 __device__ __forceinline__ int first_brick(int xcd) {
-  int blk = blockIdx.x;
-  if (xcd) {
-    const int per = gridDim.x >> 3;
-    blk = (blk & 7) * per + (blk >> 3);
-  }
-  return __builtin_amdgcn_readfirstlane((blk * kWaves + (int)(threadIdx.x >> 6)) * NB);
+  const int blk = blockIdx.x;
+  const int per = gridDim.x >> 3;
+  const int remapped = (blk & 7) * per + (blk >> 3);
+  const int b = xcd ? remapped : blk;
+  return __builtin_amdgcn_readfirstlane((b * kWaves + (int)(threadIdx.x >> 6)) * NB);
 }
 
 // Lane patterns of a brick shape: INB[d] = lanes whose neighbour in direction d
@@ -262,6 +263,10 @@ struct Stencil {
 };
 
 // Wave-uniform slot values a stencil kernel needs, loaded together.
+// LEAN kernels run when no simulation in the arena has scar, temporary-block or
+// heterogeneous-alpha voxels and every one runs the launched sweep count; they drop the
+// per-voxel alpha and the identity-sweep logic (the scalar and vector work they cost is a
+// measurable share of a sweep, which is bound by instruction issue as much as by memory).
 // Per-simulation sweep counts: a solve of K launched sweeps carries a simulation whose own
 // count is nsw < K by identity sweeps (x_{k+1} = x_k) from launch nsw on, so it computes
 // exactly what it computes alone.  Identity launches other than the last one only move the
@@ -273,10 +278,15 @@ struct SlotView {
   bool ident, skip;
   float alpha, inv_theta, ca, cb, cg;
   double alpha64;
+  template <bool LEAN = false>
   __device__ __forceinline__ void load(const SlotRec& S, int k, int K = 0, bool last = true) {
-    const int nsw = S.nsw;
-    ident = K > 0 && k >= nsw;
-    skip = ident && !last && !(k == nsw && ((K - 1 - nsw) & 1));
+    if (LEAN) {
+      ident = skip = false;
+    } else {
+      const int nsw = S.nsw;
+      ident = K > 0 && k >= nsw;
+      skip = ident && !last && !(k == nsw && ((K - 1 - nsw) & 1));
+    }
     live = S.live;
     needs_vf = S.needs_vf;
     hetero = S.hetero;
@@ -290,9 +300,10 @@ struct SlotView {
   }
 };
 
-template <typename T>
+template <typename T, bool LEAN = false>
 __device__ __forceinline__ T alpha_at(const ArenaDev& A, const SlotView& V, int idx) {
   T a = sizeof(T) == 8 ? (T)V.alpha64 : (T)V.alpha;
+  if (LEAN) return a;
   if (V.hetero) a = (T)ld(A.ALPHA, idx);
   if (V.needs_vf) {
     const uint8_t f = ld(A.VF, idx);
@@ -399,7 +410,7 @@ __global__ void __launch_bounds__(kBlock) k_reaction(ArenaDev A, int b_lo, int b
 // Right-hand side of the correction solve A e = r, x = u* + g + e, with the warm start g:
 //   r = dt L u* - A g = 2 alpha sum (u*_m - u*_n) - (g_n - alpha sum (g_m - g_n)),
 // formed in FP64 from FP32 u* and g.
-template <int BX, int BY, int BZ, int NB, int GUESS>
+template <int BX, int BY, int BZ, int NB, int GUESS, bool LEAN>
 __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
   const int b0 = first_brick<NB>(A.xcd);
   const int lane = lane_id();
@@ -419,7 +430,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
     }
     if (on[j]) {
       st[j].init(br);
-      sv[j].load(A.slots[br.slot], 0);
+      sv[j].template load<LEAN>(A.slots[br.slot], 0);
       un[j] = ld(A.U, st[j].idx);
       st[j].gather(A.U, gu[j]);
       if (GUESS >= 1) {
@@ -441,7 +452,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
       gn = gv;
     }
     if (!st[j].valid || !sv[j].live) continue;
-    const double a = alpha_at<double>(A, sv[j], st[j].idx);
+    const double a = alpha_at<double, LEAN>(A, sv[j], st[j].idx);
     stv(A.R0, st[j].idx, (float)(2.0 * a * s - (gn - a * sg)));
   }
 }
@@ -483,7 +494,7 @@ __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotRec& S
 // 2: prev[].  GUESS: warm start of the correction (see guess_array); the final modes add it
 // back and keep the step's total correction in D1 (and, with guess 2, the next step's
 // extrapolated warm start in D2).
-template <int BX, int BY, int BZ, int MODE, int NB, int GUESS, bool FIRST>
+template <int BX, int BY, int BZ, int MODE, int NB, int GUESS, bool FIRST, bool LEAN>
 __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, const float* cur,
                                                   const float* prev, int prev_kind, float* out) {
   const int b0 = first_brick<NB>(A.xcd);
@@ -507,7 +518,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       // the loads are issued before the skip test so that they depend on the brick record
       // only; a skipped wave drops them
       st[j].init(br);
-      sv[j].load(A.slots[br.slot], k, K, MODE != MODE_MID);
+      sv[j].template load<LEAN>(A.slots[br.slot], k, K, MODE != MODE_MID);
       const int i = st[j].idx;
       dn[j] = ld(src, i);
       r0[j] = ld(A.R0, i);
@@ -525,7 +536,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
     if (!st[j].valid || !sv[j].live) continue;
     const int i = st[j].idx;
     const SlotRec& S = A.slots[A.bricks[b0 + j].slot];
-    const float a = alpha_at<float>(A, sv[j], i);
+    const float a = alpha_at<float, LEAN>(A, sv[j], i);
     const float Ad = d - a * s;
     const float dpv = prev_kind == 1 ? r0[j] * inv_theta : dp[j];
     const float dnew = sv[j].ident ? d : sv[j].ca * d + sv[j].cb * dpv + sv[j].cg * (r0[j] - Ad);

@@ -87,6 +87,7 @@ struct SlotHost {
   double lmin = 1.0, lmax = 1.0;
   int32_t need_sweeps = 2;  // sweeps this slot asks of the arena (automatic mode)
   bool needs_vf = false;
+  bool hetero = false;      // per-voxel alpha
   void* staging = nullptr;  // pinned upload buffer, freed after its copies complete
   hipEvent_t upload_done = nullptr;
 };
@@ -119,7 +120,9 @@ struct phs_arena {
   int64_t g_grid = -1;
   int64_t g_high = -1;     // brick_high captured in the graph's kernel arguments
   int32_t g_sweeps = -1;
+  int32_t g_lean = -1;
   int32_t sweeps_eff = 2;  // sweeps of the first CN solve in the launched sequence
+  bool lean = false;       // no simulation needs per-voxel alpha or identity sweeps
 };
 
 namespace {
@@ -163,36 +166,47 @@ int grid_for(const phs_arena* a) {
 }
 
 // Launches of one time step for brick shape (BX, BY, BZ) and NB bricks per wavefront.
+// LEAN kernels (one brick per wavefront only) serve arenas whose simulations need neither
+// per-voxel alpha nor identity sweeps.
 template <int BX, int BY, int BZ, int NB, int GUESS>
 struct Step {
-  template <int MODE>
+  template <int MODE, bool LEAN>
   static void sweep_m(int grid, hipStream_t s, const ArenaDev& A, int k, int K, const float* cur,
                       const float* prev, int prev_kind, float* out) {
     if (k == 1)
-      k_sweep<BX, BY, BZ, MODE, NB, GUESS, true><<<grid, kBlock, 0, s>>>(A, k, K, cur, prev,
-                                                                         prev_kind, out);
+      k_sweep<BX, BY, BZ, MODE, NB, GUESS, true, LEAN><<<grid, kBlock, 0, s>>>(A, k, K, cur, prev,
+                                                                               prev_kind, out);
     else
-      k_sweep<BX, BY, BZ, MODE, NB, GUESS, false><<<grid, kBlock, 0, s>>>(A, k, K, cur, prev,
-                                                                          prev_kind, out);
+      k_sweep<BX, BY, BZ, MODE, NB, GUESS, false, LEAN><<<grid, kBlock, 0, s>>>(A, k, K, cur, prev,
+                                                                                prev_kind, out);
   }
+  template <bool LEAN>
   static void sweep(int mode, int grid, hipStream_t s, const ArenaDev& A, int k, int K,
                     const float* cur, const float* prev, int prev_kind, float* out) {
     switch (mode) {
-      case MODE_MID: sweep_m<MODE_MID>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
-      case MODE_FINAL_X: sweep_m<MODE_FINAL_X>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
-      case MODE_DACC_SET: sweep_m<MODE_DACC_SET>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
-      case MODE_DACC_ADD: sweep_m<MODE_DACC_ADD>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
-      default: sweep_m<MODE_DACC_ADD_X>(grid, s, A, k, K, cur, prev, prev_kind, out);
+      case MODE_MID: sweep_m<MODE_MID, LEAN>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
+      case MODE_FINAL_X: sweep_m<MODE_FINAL_X, LEAN>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
+      case MODE_DACC_SET: sweep_m<MODE_DACC_SET, LEAN>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
+      case MODE_DACC_ADD: sweep_m<MODE_DACC_ADD, LEAN>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
+      default: sweep_m<MODE_DACC_ADD_X, LEAN>(grid, s, A, k, K, cur, prev, prev_kind, out);
     }
   }
 
   static void run(phs_arena* a, hipStream_t s) {
+    if constexpr (NB == 1) {
+      if (a->lean) return run_t<true>(a, s);
+    }
+    run_t<false>(a, s);
+  }
+
+  template <bool LEAN>
+  static void run_t(phs_arena* a, hipStream_t s) {
     const ArenaDev A = dev_view(a);
     const int grid = grid_for(a);
     const int grid1 = grid_for_nb1(a);
     const phs_arena_config& c = a->cfg;
     // the step's reaction already ran in the previous step's last sweep (or at add)
-    k_rhs<BX, BY, BZ, NB, GUESS><<<grid, kBlock, 0, s>>>(A);
+    k_rhs<BX, BY, BZ, NB, GUESS, LEAN><<<grid, kBlock, 0, s>>>(A);
     float* bufs[2] = {a->DA, a->DB};
     // One Chebyshev solve of m sweeps on the right-hand side held in R0; per_slot: each
     // simulation runs its own count (<= m) of the first solve.
@@ -202,8 +216,8 @@ struct Step {
         const float* cur = k == 1 ? nullptr : bufs[k % 2];
         const float* prev = k <= 2 ? nullptr : bufs[(k - 1) % 2];
         const int prev_kind = k == 1 ? 0 : (k == 2 ? 1 : 2);
-        sweep(k < m - 1 ? MODE_MID : last_mode, grid, s, A, k, K, cur, prev, prev_kind,
-              bufs[(k + 1) % 2]);
+        sweep<LEAN>(k < m - 1 ? MODE_MID : last_mode, grid, s, A, k, K, cur, prev, prev_kind,
+                    bufs[(k + 1) % 2]);
       }
     };
     if (c.refine <= 0) {
@@ -220,8 +234,16 @@ struct Step {
 
   // Average duration (ms) of the reaction, rhs and sweep kernels over `reps` launches,
   // timed with HIP events on `s`.  The reaction writes into scratch (DA, DB) and the sweep
-  // runs in MID mode, so no simulation state changes.
+  // runs in MID mode, so no simulation state changes.  The kernels are the ones a step of
+  // the arena launches (LEAN or not).
   static hipError_t profile(phs_arena* a, int reps, double* ms, hipStream_t s) {
+    if constexpr (NB == 1) {
+      if (a->lean) return profile_t<true>(a, reps, ms, s);
+    }
+    return profile_t<false>(a, reps, ms, s);
+  }
+  template <bool LEAN>
+  static hipError_t profile_t(phs_arena* a, int reps, double* ms, hipStream_t s) {
     const ArenaDev A = dev_view(a);
     const int grid = grid_for(a);
     hipEvent_t ev[4];
@@ -234,11 +256,11 @@ struct Step {
     for (int i = 0; i < reps; ++i)
       k_reaction<NB><<<grid, kBlock, 0, s>>>(R, 0, (int)a->brick_high);
     hipEventRecord(ev[1], s);
-    for (int i = 0; i < reps; ++i) k_rhs<BX, BY, BZ, NB, GUESS><<<grid, kBlock, 0, s>>>(A);
+    for (int i = 0; i < reps; ++i) k_rhs<BX, BY, BZ, NB, GUESS, LEAN><<<grid, kBlock, 0, s>>>(A);
     hipEventRecord(ev[2], s);
     for (int i = 0; i < reps; ++i)
-      k_sweep<BX, BY, BZ, MODE_MID, NB, GUESS, false><<<grid, kBlock, 0, s>>>(A, 3, 0, a->DA, a->DB, 2,
-                                                                           a->DB);
+      k_sweep<BX, BY, BZ, MODE_MID, NB, GUESS, false, LEAN><<<grid, kBlock, 0, s>>>(
+          A, 3, 0, a->DA, a->DB, 2, a->DB);
     hipEventRecord(ev[3], s);
     hipError_t r = hipEventSynchronize(ev[3]);
     for (int i = 0; i < 3; ++i) {
@@ -359,6 +381,10 @@ void recompute_high(phs_arena* a) {
     }
   a->brick_high = hi;
   a->sweeps_eff = a->cfg.sweeps > 0 ? a->cfg.sweeps : sw;
+  bool lean = true;
+  for (auto& h : a->sh)
+    if (h.used && (h.needs_vf || h.hetero || h.need_sweeps != a->sweeps_eff)) lean = false;
+  a->lean = lean;
 }
 
 }  // namespace
@@ -561,6 +587,7 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
   h.lmin = 1.0;
   h.lmax = 1.0 + 2.0 * amax_deg;
   h.needs_vf = needs_vf;
+  h.hetero = hetero;
   {
     // Inexact CN steps amplify the stiffest modes unless the sweep polynomial is small
     // enough there; reject a fixed count that would, or pick one (automatic mode).
@@ -701,7 +728,7 @@ int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   const int64_t grid = grid_for(a);
   if (!a->gexec || a->g_steps != n_steps || a->g_grid != grid || a->g_high != a->brick_high ||
-      a->g_sweeps != a->sweeps_eff) {
+      a->g_sweeps != a->sweeps_eff || a->g_lean != (int32_t)a->lean) {
     if (a->gexec) hipGraphExecDestroy(a->gexec);
     if (a->graph) hipGraphDestroy(a->graph);
     a->gexec = nullptr;
@@ -717,6 +744,7 @@ int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream) {
     a->g_grid = grid;
     a->g_high = a->brick_high;
     a->g_sweeps = a->sweeps_eff;
+    a->g_lean = a->lean ? 1 : 0;
   }
   PHS_HIP_CHECK(hipGraphLaunch(a->gexec, s));
   for (auto& h : a->sh)

@@ -84,6 +84,8 @@ def test_fp32_arena_lat_within_one_step(cuda):
 
 
 def test_arena_alone_equals_mixed_and_graph(cuda):
+    """A simulation gives the same bits alone (lean kernels: no scar, one sweep count) as
+    packed with a scarred one (general kernels), admitted mid-run, with graph replay."""
     from pyheartsim_amd import runtime
     wa = small_shell(t_final=20.0, seed=3)
     wb = small_shell(t_final=30.0, seed=5, semi_axes=(8.0, 7.0, 6.0), n_scar=1)
