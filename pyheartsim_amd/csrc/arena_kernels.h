@@ -358,18 +358,53 @@ __device__ __forceinline__ void rk4_model(const float (&p)[5], float dt, float J
 // One RK4 step of the cell model (simulation/simulation_gpu.py:71-130) in FP32; the stimulus
 // J is 1 on the S1 (S2) sites while the schedule byte sc has the S1 (S2) bit.  The model is
 // wave-uniform: one scalar branch per call.
-__device__ __forceinline__ void rk4(const SlotRec& S, int sc, uint8_t f, float u, float h,
-                                    float& uo, float& ho) {
+__device__ __forceinline__ void rk4(const float (&p)[5], float dt, int model, int sc, uint8_t f,
+                                    float u, float h, float& uo, float& ho) {
   const float J = (((sc & PHS_SCHED_S1) && (f & PHS_VF_S1)) ||
                    ((sc & PHS_SCHED_S2) && (f & PHS_VF_S2)))
                       ? 1.0f
                       : 0.0f;
-  const float p[5] = {S.p[0], S.p[1], S.p[2], S.p[3], S.p[4]};
-  if (S.model == 0)
-    rk4_model<0>(p, S.dt, J, u, h, uo, ho);
+  if (model == 0)
+    rk4_model<0>(p, dt, J, u, h, uo, ho);
   else
-    rk4_model<1>(p, S.dt, J, u, h, uo, ho);
+    rk4_model<1>(p, dt, J, u, h, uo, ho);
 }
+__device__ __forceinline__ void rk4(const SlotRec& S, int sc, uint8_t f, float u, float h,
+                                    float& uo, float& ho) {
+  const float p[5] = {S.p[0], S.p[1], S.p[2], S.p[3], S.p[4]};
+  rk4(p, S.dt, S.model, sc, f, u, h, uo, ho);
+}
+
+// The slot values the last sweep needs to finish a voxel.  They are read with the other
+// wave-uniform values, before the kernel's first store: the stores could alias the slot
+// table as far as the compiler knows, so reads after them become vector loads, each a
+// memory round trip of its own.
+struct SlotFinish {
+  int64_t step, n_steps, voxel_off, cur_off;
+  float* su;
+  float* sh;
+  int32_t* lat;
+  int32_t n_voxel, max_beats, next_sched, model;
+  float p[5];
+  float dt, lat_thr;
+  __device__ __forceinline__ void load(const SlotRec& S) {
+    step = S.step;
+    n_steps = S.n_steps;
+    voxel_off = S.voxel_off;
+    cur_off = S.cur_off;
+    su = S.su;
+    sh = S.sh;
+    lat = S.lat;
+    n_voxel = S.n_voxel;
+    max_beats = S.max_beats;
+    next_sched = S.next_sched;
+    model = S.model;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) p[i] = S.p[i];
+    dt = S.dt;
+    lat_thr = S.lat_thr;
+  }
+};
 
 // The reaction of a step on its own: the first step of a newly added simulation (bricks
 // [b_lo, b_hi) only), and the timing of the reaction in isolation.  Every later step's
@@ -459,10 +494,10 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
 
 // Last pass of a step: activation time and samples of the post-step state x, then the next
 // step's reaction, so that U holds u* of the next step between steps (x itself after a
-// simulation's last step).
-__device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotRec& S, int idx, float x) {
-  const uint8_t f = A.VF[idx];
-  const float hv = A.H[idx];
+// simulation's last step).  f and hv are the voxel's flags and gate, loaded by the caller
+// together with its stencil values.
+__device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotFinish& S, int idx,
+                                             float x, uint8_t f, float hv) {
   const bool now = x >= S.lat_thr;
   const bool above = (f & PHS_VF_ABOVE) != 0;
   const int64_t local = (int64_t)idx - S.voxel_off;
@@ -481,7 +516,7 @@ __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotRec& S
   }
   if (S.step + 1 < S.n_steps) {
     float uo, ho;
-    rk4(S, S.next_sched, f, x, hv, uo, ho);
+    rk4(S.p, S.dt, S.model, S.next_sched, f, x, hv, uo, ho);
     A.U[idx] = uo;
     A.H[idx] = ho;
   } else {
@@ -505,6 +540,12 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
   bool on[NB];
   float dn[NB], r0[NB], dp[NB];
   float g6[NB][6];
+  // the last sweep's own-voxel state (u*, h, flags, previous corrections), loaded with the
+  // stencil values so that finishing the step waits on no second round trip
+  constexpr bool kFinish = MODE == MODE_FINAL_X || MODE == MODE_DACC_ADD_X;
+  float pu[NB], ph[NB], p1[NB], p2[NB];
+  uint8_t pf[NB];
+  SlotFinish sf[NB];
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int b = b0 + j;
@@ -524,6 +565,14 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       r0[j] = ld(A.R0, i);
       dp[j] = prev_kind == 2 ? ld(prev, i) : 0.0f;
       st[j].gather(src, g6[j]);
+      if (kFinish) {
+        sf[j].load(A.slots[br.slot]);
+        pu[j] = ld(A.U, i);
+        ph[j] = ld(A.H, i);
+        pf[j] = ld(A.VF, i);
+        p1[j] = GUESS >= 1 ? ld(A.D1, i) : 0.0f;
+        p2[j] = GUESS == 2 ? ld(A.D2, i) : 0.0f;
+      }
     }
   }
 #pragma unroll
@@ -535,7 +584,6 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
     const float s = st[j].template sum<float>(d, g6[j], sc, lane);
     if (!st[j].valid || !sv[j].live) continue;
     const int i = st[j].idx;
-    const SlotRec& S = A.slots[A.bricks[b0 + j].slot];
     const float a = alpha_at<float, LEAN>(A, sv[j], i);
     const float Ad = d - a * s;
     const float dpv = prev_kind == 1 ? r0[j] * inv_theta : dp[j];
@@ -543,18 +591,18 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
     if (MODE == MODE_MID) {
       stv(out, i, dnew);
     } else if (MODE == MODE_FINAL_X) {
-      const float us = ld(A.U, i);
+      const float us = pu[j];
       float dt = dnew;
       if (GUESS >= 1) {
-        const float o1 = ld(A.D1, i);
-        dt += GUESS == 2 ? ld(A.D2, i) : o1;
+        const float o1 = p1[j];
+        dt += GUESS == 2 ? p2[j] : o1;
         keep_correction<GUESS>(A, i, o1, dt);
       }
       if (A.diag) {
         A.USTAR[i] = us;
         A.DACC[i] = (double)dt;
       }
-      finish_voxel(A, S, i, us + dt);
+      finish_voxel(A, sf[j], i, us + dt, pf[j], ph[j]);
     } else if (MODE == MODE_DACC_SET) {
       double base = 0.0;
       if (GUESS >= 1) base = guess_array<GUESS>(A)[i];
@@ -563,11 +611,11 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       A.DACC[i] += (double)dnew;
     } else {
       const double acc = A.DACC[i] + (double)dnew;
-      const float us = A.U[i];
+      const float us = pu[j];
       A.DACC[i] = acc;
-      if (GUESS >= 1) keep_correction<GUESS>(A, i, A.D1[i], (float)acc);
+      if (GUESS >= 1) keep_correction<GUESS>(A, i, p1[j], (float)acc);
       if (A.diag) A.USTAR[i] = us;
-      finish_voxel(A, S, i, (float)((double)us + acc));
+      finish_voxel(A, sf[j], i, (float)((double)us + acc), pf[j], ph[j]);
     }
   }
 }
