@@ -300,13 +300,29 @@ struct SlotView {
   }
 };
 
+// The per-voxel inputs of alpha (a heterogeneous value, the scar and temporary-block flags),
+// loaded in a kernel's load phase when the simulation has them, so that they travel with the
+// stencil loads instead of waiting behind them.
+struct AlphaIn {
+  float al;
+  uint8_t f;
+  template <bool LEAN>
+  __device__ __forceinline__ void load(const ArenaDev& A, const SlotView& V, int idx) {
+    al = 0.0f;
+    f = 0;
+    if (LEAN) return;
+    if (V.hetero) al = ld(A.ALPHA, idx);
+    if (V.needs_vf) f = ld(A.VF, idx);
+  }
+};
+
 template <typename T, bool LEAN = false>
-__device__ __forceinline__ T alpha_at(const ArenaDev& A, const SlotView& V, int idx) {
+__device__ __forceinline__ T alpha_at(const SlotView& V, const AlphaIn& in) {
   T a = sizeof(T) == 8 ? (T)V.alpha64 : (T)V.alpha;
   if (LEAN) return a;
-  if (V.hetero) a = (T)ld(A.ALPHA, idx);
+  if (V.hetero) a = (T)in.al;
   if (V.needs_vf) {
-    const uint8_t f = ld(A.VF, idx);
+    const uint8_t f = in.f;
     if ((f & PHS_VF_SCAR) || ((V.cur_sched & PHS_SCHED_TEMP_BLOCK) && (f & PHS_VF_TEMP_BLOCK)))
       a = (T)0;
   }
@@ -454,6 +470,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
   bool on[NB];
   float un[NB], gn0[NB];
   float gu[NB][6], gg[NB][6];
+  AlphaIn ain[NB];
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int b = b0 + j;
@@ -473,6 +490,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
         gn0[j] = ld(G, st[j].idx);
         st[j].gather(G, gg[j]);
       }
+      ain[j].template load<LEAN>(A, sv[j], st[j].idx);
     }
   }
 #pragma unroll
@@ -487,7 +505,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
       gn = gv;
     }
     if (!st[j].valid || !sv[j].live) continue;
-    const double a = alpha_at<double, LEAN>(A, sv[j], st[j].idx);
+    const double a = alpha_at<double, LEAN>(sv[j], ain[j]);
     stv(A.R0, st[j].idx, (float)(2.0 * a * s - (gn - a * sg)));
   }
 }
@@ -546,6 +564,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
   float pu[NB], ph[NB], p1[NB], p2[NB];
   uint8_t pf[NB];
   SlotFinish sf[NB];
+  AlphaIn ain[NB];
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int b = b0 + j;
@@ -565,6 +584,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       r0[j] = ld(A.R0, i);
       dp[j] = prev_kind == 2 ? ld(prev, i) : 0.0f;
       st[j].gather(src, g6[j]);
+      ain[j].template load<LEAN>(A, sv[j], i);
       if (kFinish) {
         sf[j].load(A.slots[br.slot]);
         pu[j] = ld(A.U, i);
@@ -584,7 +604,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
     const float s = st[j].template sum<float>(d, g6[j], sc, lane);
     if (!st[j].valid || !sv[j].live) continue;
     const int i = st[j].idx;
-    const float a = alpha_at<float, LEAN>(A, sv[j], i);
+    const float a = alpha_at<float, LEAN>(sv[j], ain[j]);
     const float Ad = d - a * s;
     const float dpv = prev_kind == 1 ? r0[j] * inv_theta : dp[j];
     const float dnew = sv[j].ident ? d : sv[j].ca * d + sv[j].cb * dpv + sv[j].cg * (r0[j] - Ad);
