@@ -663,25 +663,36 @@ __global__ void __launch_bounds__(kBlock) k_residual(ArenaDev A) {
 }
 
 // Advance the per-simulation counters and stage the next step's schedule byte and sample row.
+// Every read comes before the first store: the stores could alias the slot fields as far as
+// the compiler knows, and a read after one would wait for it (a chain of round trips).
 __global__ void k_advance(SlotRec* slots, int n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n || !slots[s].live) return;
+  if (s >= n) return;
   SlotRec& S = slots[s];
+  const int live = S.live;
   const int64_t t = S.step + 1;
-  S.step = t;
-  if (t >= S.n_steps) {
-    S.live = 0;
-    return;
-  }
-  S.cur_sched = S.sched[t];
-  S.next_sched = t + 1 < S.n_steps ? S.sched[t + 1] : 0;
+  const int64_t n_steps = S.n_steps;
+  const uint8_t* sched = S.sched;
+  const int nspm = S.nspm, n_samples = S.n_samples, ring = S.ring, n_voxel = S.n_voxel;
+  if (!live) return;
+  const bool more = t < n_steps;
+  const int cur = more ? sched[t] : 0;
+  const int next = t + 1 < n_steps ? sched[t + 1] : 0;
   int64_t row = -1;
-  if (S.nspm > 0) {
-    if (t % S.nspm == 0) row = t / S.nspm;
+  if (nspm > 0) {
+    if (t % nspm == 0) row = t / nspm;
   } else {
     row = t;
   }
-  S.cur_off = row >= 0 && row < S.n_samples ? (S.ring > 0 ? row % S.ring : row) * S.n_voxel : -1;
+  const int64_t off = row >= 0 && row < n_samples ? (ring > 0 ? row % ring : row) * n_voxel : -1;
+  S.step = t;
+  if (!more) {
+    S.live = 0;
+    return;
+  }
+  S.cur_sched = cur;
+  S.next_sched = next;
+  S.cur_off = off;
 }
 
 __global__ void k_deactivate(SlotRec* slots, int slot) { slots[slot].live = 0; }
