@@ -329,18 +329,6 @@ __device__ __forceinline__ T alpha_at(const SlotView& V, const AlphaIn& in) {
   return a;
 }
 
-template <typename T>
-__device__ __forceinline__ T alpha_of(const ArenaDev& A, const SlotRec& S, int idx) {
-  T a = sizeof(T) == 8 ? (T)S.alpha64 : (T)S.alpha;
-  if (S.hetero) a = (T)ld(A.ALPHA, idx);
-  if (S.needs_vf) {
-    const uint8_t f = ld(A.VF, idx);
-    if ((f & PHS_VF_SCAR) || ((S.cur_sched & PHS_SCHED_TEMP_BLOCK) && (f & PHS_VF_TEMP_BLOCK)))
-      a = (T)0;
-  }
-  return a;
-}
-
 // Cell-model right-hand side (simulation/simulation_gpu.py:27-43) in FP32, branch-free:
 // MODEL 0 Mitchell-Schaeffer (p = 1/tau_open, 1/tau_close, 1/tau_in, 1/tau_out, v_gate),
 // MODEL 1 Aliev-Panfilov (p = k, a, eps0, mu1, mu2).
@@ -565,6 +553,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
   uint8_t pf[NB];
   SlotFinish sf[NB];
   AlphaIn ain[NB];
+  // the refinement modes' accumulator (or warm start), likewise loaded up front
+  double pd[NB];
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int b = b0 + j;
@@ -585,6 +575,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       dp[j] = prev_kind == 2 ? ld(prev, i) : 0.0f;
       st[j].gather(src, g6[j]);
       ain[j].template load<LEAN>(A, sv[j], i);
+      if (MODE == MODE_DACC_ADD || MODE == MODE_DACC_ADD_X) pd[j] = A.DACC[i];
+      if (MODE == MODE_DACC_SET) pd[j] = GUESS >= 1 ? (double)guess_array<GUESS>(A)[i] : 0.0;
       if (kFinish) {
         sf[j].load(A.slots[br.slot]);
         pu[j] = ld(A.U, i);
@@ -624,13 +616,11 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       }
       finish_voxel(A, sf[j], i, us + dt, pf[j], ph[j]);
     } else if (MODE == MODE_DACC_SET) {
-      double base = 0.0;
-      if (GUESS >= 1) base = guess_array<GUESS>(A)[i];
-      A.DACC[i] = base + (double)dnew;
+      A.DACC[i] = pd[j] + (double)dnew;
     } else if (MODE == MODE_DACC_ADD) {
-      A.DACC[i] += (double)dnew;
+      A.DACC[i] = pd[j] + (double)dnew;
     } else {
-      const double acc = A.DACC[i] + (double)dnew;
+      const double acc = pd[j] + (double)dnew;
       const float us = pu[j];
       A.DACC[i] = acc;
       if (GUESS >= 1) keep_correction<GUESS>(A, i, p1[j], (float)acc);
@@ -648,6 +638,10 @@ __global__ void __launch_bounds__(kBlock) k_residual(ArenaDev A) {
   const int lane = lane_id();
   Stencil<BX, BY, BZ> st;
   st.init(A.bricks[b]);
+  SlotView sv;
+  sv.load(A.slots[A.bricks[b].slot], 0);
+  AlphaIn ain;
+  ain.load<false>(A, sv, st.idx);
   const float un = A.U[st.idx];
   const double dn = A.DACC[st.idx];
   float gu[6];
@@ -656,9 +650,8 @@ __global__ void __launch_bounds__(kBlock) k_residual(ArenaDev A) {
   st.gather((const double*)A.DACC, gd);
   const double su = st.template sum<double>(st.valid ? un : 0.0f, gu, 1.0f, lane);
   const double sd = st.template sum<double>(st.valid ? dn : 0.0, gd, 1.0, lane);
-  const SlotRec& S = A.slots[A.bricks[b].slot];
-  if (!st.valid || !S.live) return;
-  const double a = alpha_of<double>(A, S, st.idx);
+  if (!st.valid || !sv.live) return;
+  const double a = alpha_at<double>(sv, ain);
   A.R0[st.idx] = (float)(2.0 * a * su - (dn - a * sd));
 }
 
