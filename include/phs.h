@@ -256,7 +256,8 @@ int phs_egm_electrode_lat(const float* egm, int64_t n_time, int32_t n_electrode,
 /* Activation markers and local activation times of sampled action potentials (replaces
  * legacy/calculate_local_activation_time.py:18-66 execute_on_action_potential with the
  * markers of :67-136 find_action_potential_marker).  ap: device float, sample t of signal i at
- * ap[t * stride + i] (stride = n_signal for a row-major [n_time][n_signal] array).  Markers of a signal: for each pair of the sorted unique boundary points
+ * ap[t * stride + i] (stride = n_signal for a row-major [n_time][n_signal] array).  Markers
+ * of a signal: for each pair of the sorted unique boundary points
  * of its runs above `threshold`, the first index of the largest forward difference, kept when
  * that difference is > 0 (or NaN).  lat[i] = the first marker in [t_start, t_start +
  * round(CL * cycle_length_fraction)] (CL = mean cycle length over all signals, 180 if none),
