@@ -42,21 +42,30 @@ struct BrickRec {
 };
 static_assert(sizeof(BrickRec) == 192, "brick record layout");
 
-struct SlotRec {
+// The values of a simulation that change from step to step.  A slot keeps two copies: the
+// kernels of a step read copy `par` (the parity of the arena's launched-step count), and the
+// step's first kernel writes the next step's values into copy par ^ 1, which no kernel of the
+// step reads, so no separate advance launch is needed.
+struct SlotDyn {
   int64_t step;       // steps done
-  int64_t n_steps;
-  int64_t voxel_off;
   int64_t cur_off;    // element offset of the sample row written by this step in the sample
                       // buffers (row modulo the ring length, times n_voxel), -1 = none
+  int32_t live;       // resident and step < n_steps
+  int32_t cur_sched;  // schedule byte of this step
+  int32_t next_sched; // schedule byte of the next step (its reaction runs in this step's
+                      // last sweep), 0 after the last step
+  int32_t pad;
+};
+
+struct SlotRec {
+  SlotDyn dyn[2];
+  int64_t n_steps;
+  int64_t voxel_off;
   const uint8_t* sched;
   float* su;
   float* sh;
   int32_t* lat;
   int32_t n_voxel;
-  int32_t live;       // resident and step < n_steps
-  int32_t cur_sched;  // schedule byte of this step
-  int32_t next_sched; // schedule byte of the next step (its reaction runs in this step's
-                      // last sweep), 0 after the last step
   int32_t model;
   int32_t nspm;
   int32_t n_samples;
@@ -89,9 +98,15 @@ struct ArenaDev {
   float* D2;  // guess 2: the next step's warm start 2 D1 - (correction of the step before),
               // formed in FP32 when D1 is written
   int32_t n_bricks;
-  int32_t xcd;
+  int32_t flags;     // bit 0: XCD remap; bit 1: launch parity of this step (the slots' copy
+                     // of the per-step values).  One word with n_bricks, so that every kernel
+                     // has both from its first kernel-argument load.
   int32_t diag;
+  int32_t max_sims;  // slots to advance
 };
+
+__device__ __forceinline__ int xcd_of(const ArenaDev& A) { return A.flags & 1; }
+__device__ __forceinline__ int par_of(const ArenaDev& A) { return (A.flags >> 1) & 1; }
 
 enum { MODE_MID = 0, MODE_FINAL_X = 1, MODE_DACC_SET = 2, MODE_DACC_ADD = 3, MODE_DACC_ADD_X = 4 };
 
@@ -279,7 +294,8 @@ struct SlotView {
   float alpha, inv_theta, ca, cb, cg;
   double alpha64;
   template <bool LEAN = false>
-  __device__ __forceinline__ void load(const SlotRec& S, int k, int K = 0, bool last = true) {
+  __device__ __forceinline__ void load(const SlotRec& S, int par, int k, int K = 0,
+                                       bool last = true) {
     if (LEAN) {
       ident = skip = false;
     } else {
@@ -287,10 +303,10 @@ struct SlotView {
       ident = K > 0 && k >= nsw;
       skip = ident && !last && !(k == nsw && ((K - 1 - nsw) & 1));
     }
-    live = S.live;
+    live = S.dyn[par].live;
     needs_vf = S.needs_vf;
     hetero = S.hetero;
-    cur_sched = S.cur_sched;
+    cur_sched = S.dyn[par].cur_sched;
     alpha = S.alpha;
     alpha64 = S.alpha64;
     inv_theta = S.coef[0][2];
@@ -391,17 +407,17 @@ struct SlotFinish {
   int32_t n_voxel, max_beats, next_sched, model;
   float p[5];
   float dt, lat_thr;
-  __device__ __forceinline__ void load(const SlotRec& S) {
-    step = S.step;
+  __device__ __forceinline__ void load(const SlotRec& S, int par) {
+    step = S.dyn[par].step;
     n_steps = S.n_steps;
     voxel_off = S.voxel_off;
-    cur_off = S.cur_off;
+    cur_off = S.dyn[par].cur_off;
     su = S.su;
     sh = S.sh;
     lat = S.lat;
     n_voxel = S.n_voxel;
     max_beats = S.max_beats;
-    next_sched = S.next_sched;
+    next_sched = S.dyn[par].next_sched;
     model = S.model;
 #pragma unroll
     for (int i = 0; i < 5; ++i) p[i] = S.p[i];
@@ -416,7 +432,7 @@ struct SlotFinish {
 // record alone; the slot's liveness is checked before anything is stored.
 template <int NB>
 __global__ void __launch_bounds__(kBlock) k_reaction(ArenaDev A, int b_lo, int b_hi) {
-  const int b0 = first_brick<NB>(A.xcd) + b_lo;
+  const int b0 = first_brick<NB>(xcd_of(A)) + b_lo;
   float u[NB], h[NB];
   uint8_t f[NB];
   int idx[NB];
@@ -438,12 +454,42 @@ __global__ void __launch_bounds__(kBlock) k_reaction(ArenaDev A, int b_lo, int b
   for (int j = 0; j < NB; ++j) {
     if (!on[j]) continue;
     const SlotRec& S = A.slots[A.bricks[b0 + j].slot];
-    if (!S.live || !valid[j]) continue;
+    if (!S.dyn[par_of(A)].live || !valid[j]) continue;
     float uo, ho;
-    rk4(S, S.cur_sched, f[j], u[j], h[j], uo, ho);
+    rk4(S, S.dyn[par_of(A)].cur_sched, f[j], u[j], h[j], uo, ho);
     stv(A.UO, idx[j], uo);
     stv(A.HO, idx[j], ho);
   }
+}
+
+// The next step's per-step values of slot S (copy par ^ 1) from this step's (copy par): the
+// step counter, the schedule bytes and the sample row.  A finished or unused slot's values are
+// carried over unchanged, so that its copies never disagree.  Every read comes before the
+// first store.
+__device__ __forceinline__ void advance_slot(SlotRec& S, int par) {
+  const SlotDyn cur = S.dyn[par];
+  const int64_t t = cur.step + 1;
+  const int64_t n_steps = S.n_steps;
+  const uint8_t* sched = S.sched;
+  const int nspm = S.nspm, n_samples = S.n_samples, ring = S.ring, n_voxel = S.n_voxel;
+  SlotDyn nx = cur;
+  if (cur.live) {
+    const bool more = t < n_steps;
+    nx.step = t;
+    nx.live = more ? 1 : 0;
+    if (more) {
+      nx.cur_sched = sched[t];
+      nx.next_sched = t + 1 < n_steps ? sched[t + 1] : 0;
+      int64_t row = -1;
+      if (nspm > 0) {
+        if (t % nspm == 0) row = t / nspm;
+      } else {
+        row = t;
+      }
+      nx.cur_off = row >= 0 && row < n_samples ? (ring > 0 ? row % ring : row) * n_voxel : -1;
+    }
+  }
+  S.dyn[par ^ 1] = nx;
 }
 
 // Right-hand side of the correction solve A e = r, x = u* + g + e, with the warm start g:
@@ -451,7 +497,7 @@ __global__ void __launch_bounds__(kBlock) k_reaction(ArenaDev A, int b_lo, int b
 // formed in FP64 from FP32 u* and g.
 template <int BX, int BY, int BZ, int NB, int GUESS, bool LEAN>
 __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
-  const int b0 = first_brick<NB>(A.xcd);
+  const int b0 = first_brick<NB>(xcd_of(A));
   const int lane = lane_id();
   Stencil<BX, BY, BZ> st[NB];
   SlotView sv[NB];
@@ -470,7 +516,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
     }
     if (on[j]) {
       st[j].init(br);
-      sv[j].template load<LEAN>(A.slots[br.slot], 0);
+      sv[j].template load<LEAN>(A.slots[br.slot], par_of(A), 0);
       un[j] = ld(A.U, st[j].idx);
       st[j].gather(A.U, gu[j]);
       if (GUESS >= 1) {
@@ -496,6 +542,11 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
     const double a = alpha_at<double, LEAN>(sv[j], ain[j]);
     stv(A.R0, st[j].idx, (float)(2.0 * a * s - (gn - a * sg)));
   }
+  // The first workgroup's first wavefront also stages every slot's next-step values.  This
+  // comes after all of the kernel's loads: a store to the slot table before them would leave
+  // the compiler unable to keep the brick records on the scalar path.
+  if (blockIdx.x == 0 && threadIdx.x < 64)
+    for (int s = (int)threadIdx.x; s < A.max_sims; s += 64) advance_slot(A.slots[s], par_of(A));
 }
 
 // Last pass of a step: activation time and samples of the post-step state x, then the next
@@ -538,7 +589,7 @@ __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotFinish
 template <int BX, int BY, int BZ, int MODE, int NB, int GUESS, bool FIRST, bool LEAN>
 __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, const float* cur,
                                                   const float* prev, int prev_kind, float* out) {
-  const int b0 = first_brick<NB>(A.xcd);
+  const int b0 = first_brick<NB>(xcd_of(A));
   const int lane = lane_id();
   const float* src = FIRST ? A.R0 : cur;
   Stencil<BX, BY, BZ> st[NB];
@@ -568,7 +619,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       // the loads are issued before the skip test so that they depend on the brick record
       // only; a skipped wave drops them
       st[j].init(br);
-      sv[j].template load<LEAN>(A.slots[br.slot], k, K, MODE != MODE_MID);
+      sv[j].template load<LEAN>(A.slots[br.slot], par_of(A), k, K, MODE != MODE_MID);
       const int i = st[j].idx;
       dn[j] = ld(src, i);
       r0[j] = ld(A.R0, i);
@@ -578,7 +629,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       if (MODE == MODE_DACC_ADD || MODE == MODE_DACC_ADD_X) pd[j] = A.DACC[i];
       if (MODE == MODE_DACC_SET) pd[j] = GUESS >= 1 ? (double)guess_array<GUESS>(A)[i] : 0.0;
       if (kFinish) {
-        sf[j].load(A.slots[br.slot]);
+        sf[j].load(A.slots[br.slot], par_of(A));
         pu[j] = ld(A.U, i);
         ph[j] = ld(A.H, i);
         pf[j] = ld(A.VF, i);
@@ -633,13 +684,13 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
 // FP64 residual of the accumulated correction: r = 2 alpha sum(u*_m - u*_n) - A dacc.
 template <int BX, int BY, int BZ>
 __global__ void __launch_bounds__(kBlock) k_residual(ArenaDev A) {
-  const int b = first_brick<1>(A.xcd);
+  const int b = first_brick<1>(xcd_of(A));
   if (b >= A.n_bricks || A.bricks[b].slot < 0) return;
   const int lane = lane_id();
   Stencil<BX, BY, BZ> st;
   st.init(A.bricks[b]);
   SlotView sv;
-  sv.load(A.slots[A.bricks[b].slot], 0);
+  sv.load(A.slots[A.bricks[b].slot], par_of(A), 0);
   AlphaIn ain;
   ain.load<false>(A, sv, st.idx);
   const float un = A.U[st.idx];
@@ -655,40 +706,10 @@ __global__ void __launch_bounds__(kBlock) k_residual(ArenaDev A) {
   A.R0[st.idx] = (float)(2.0 * a * su - (dn - a * sd));
 }
 
-// Advance the per-simulation counters and stage the next step's schedule byte and sample row.
-// Every read comes before the first store: the stores could alias the slot fields as far as
-// the compiler knows, and a read after one would wait for it (a chain of round trips).
-__global__ void k_advance(SlotRec* slots, int n) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  SlotRec& S = slots[s];
-  const int live = S.live;
-  const int64_t t = S.step + 1;
-  const int64_t n_steps = S.n_steps;
-  const uint8_t* sched = S.sched;
-  const int nspm = S.nspm, n_samples = S.n_samples, ring = S.ring, n_voxel = S.n_voxel;
-  if (!live) return;
-  const bool more = t < n_steps;
-  const int cur = more ? sched[t] : 0;
-  const int next = t + 1 < n_steps ? sched[t + 1] : 0;
-  int64_t row = -1;
-  if (nspm > 0) {
-    if (t % nspm == 0) row = t / nspm;
-  } else {
-    row = t;
-  }
-  const int64_t off = row >= 0 && row < n_samples ? (ring > 0 ? row % ring : row) * n_voxel : -1;
-  S.step = t;
-  if (!more) {
-    S.live = 0;
-    return;
-  }
-  S.cur_sched = cur;
-  S.next_sched = next;
-  S.cur_off = off;
+__global__ void k_deactivate(SlotRec* slots, int slot) {
+  slots[slot].dyn[0].live = 0;
+  slots[slot].dyn[1].live = 0;
 }
-
-__global__ void k_deactivate(SlotRec* slots, int slot) { slots[slot].live = 0; }
 
 __global__ void k_invalidate(BrickRec* bricks, int64_t b0, int64_t nb) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -699,7 +720,7 @@ __global__ void k_invalidate(BrickRec* bricks, int64_t b0, int64_t nb) {
 // r = b - (x - alpha sum(x_m - x_n)), for the last completed step of one slot.
 template <int BX, int BY, int BZ>
 __global__ void __launch_bounds__(kBlock) k_diag(ArenaDev A, int slot_sel, double* acc) {
-  const int b = first_brick<1>(A.xcd);
+  const int b = first_brick<1>(xcd_of(A));
   if (b >= A.n_bricks || A.bricks[b].slot != slot_sel) return;
   const SlotRec& S = A.slots[slot_sel];
   const int lane = lane_id();
@@ -715,7 +736,8 @@ __global__ void __launch_bounds__(kBlock) k_diag(ArenaDev A, int slot_sel, doubl
   const double sd = st.template sum<double>(dn, gd, 1.0, lane);
   double r2 = 0.0, b2 = 0.0;
   if (st.valid) {
-    const uint8_t sc = S.step > 0 ? S.sched[S.step - 1] : 0;
+    const int64_t done = S.dyn[par_of(A)].step;
+    const uint8_t sc = done > 0 ? S.sched[done - 1] : 0;
     double a = S.hetero ? (double)A.ALPHA[st.idx] : S.alpha64;
     if (S.needs_vf) {
       const uint8_t f = A.VF[st.idx];

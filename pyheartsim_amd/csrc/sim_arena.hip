@@ -114,13 +114,18 @@ struct phs_arena {
   RangeAlloc vox, brk;
   int64_t brick_high = 0;
   hipStream_t cap_stream = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
-  int32_t g_steps = -1;
-  int64_t g_grid = -1;
-  int64_t g_high = -1;     // brick_high captured in the graph's kernel arguments
-  int32_t g_sweeps = -1;
-  int32_t g_lean = -1;
+  // One captured sequence of steps per launch parity at its first step (the kernels' slot
+  // copy alternates with the step), so that odd step counts replay without recapture too.
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int32_t steps = -1;
+    int64_t grid = -1;
+    int64_t high = -1;  // brick_high captured in the kernel arguments
+    int32_t sweeps = -1;
+    int32_t lean = -1;
+  } g[2];
+  int32_t par = 0;         // launch parity of the next step (flips with every launched step)
   int32_t sweeps_eff = 2;  // sweeps of the first CN solve in the launched sequence
   bool lean = false;       // no simulation needs per-voxel alpha or identity sweeps
 };
@@ -143,8 +148,9 @@ ArenaDev dev_view(const phs_arena* a) {
   A.D1 = a->D1;
   A.D2 = a->D2;
   A.n_bricks = (int32_t)a->brick_high;
-  A.xcd = a->cfg.xcd_remap ? 1 : 0;
+  A.flags = (a->cfg.xcd_remap ? 1 : 0) | (a->par << 1);
   A.diag = a->cfg.diag ? 1 : 0;
+  A.max_sims = a->cfg.max_sims;
   return A;
 }
 
@@ -229,7 +235,8 @@ struct Step {
         solve(c.refine_sweeps, r == c.refine - 1 ? MODE_DACC_ADD_X : MODE_DACC_ADD, false);
       }
     }
-    k_advance<<<(c.max_sims + 63) / 64, 64, 0, s>>>(a->slots, c.max_sims);
+    // the next step's per-step values were staged by this step's k_rhs
+    a->par ^= 1;
   }
 
   // Average duration (ms) of the reaction, rhs and sweep kernels over `reps` launches,
@@ -275,7 +282,7 @@ struct Step {
   // Reaction of step 0 for the bricks [b_lo, b_hi) of a newly added simulation.
   static void react_range(phs_arena* a, int b_lo, int b_hi, hipStream_t s) {
     ArenaDev A = dev_view(a);
-    A.xcd = 0;
+    A.flags &= ~1;  // no XCD remap: the grid covers a sub-range
     const int per = kWaves * NB;
     const int grid = (b_hi - b_lo + per - 1) / per;
     if (grid > 0) k_reaction<NB><<<grid, kBlock, 0, s>>>(A, b_lo, b_hi);
@@ -622,17 +629,21 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
     sr.coef[k][2] = (float)coef[k].g;
     sr.coef[k][3] = 0.0f;
   }
-  sr.step = 0;
+  SlotDyn d0;
+  std::memset(&d0, 0, sizeof(d0));
+  d0.step = 0;
   sr.n_steps = d->n_steps;
-  sr.cur_sched = d->n_steps > 0 ? d->schedule[0] : 0;
-  sr.next_sched = d->n_steps > 1 ? d->schedule[1] : 0;
-  sr.cur_off = d->n_samples > 0 ? 0 : -1;  // step 0 writes sample row 0
+  d0.cur_sched = d->n_steps > 0 ? d->schedule[0] : 0;
+  d0.next_sched = d->n_steps > 1 ? d->schedule[1] : 0;
+  d0.cur_off = d->n_samples > 0 ? 0 : -1;  // step 0 writes sample row 0
   sr.voxel_off = vo;
   sr.su = out ? (float*)out->samples_u : nullptr;
   sr.sh = out ? (float*)out->samples_h : nullptr;
   sr.lat = out ? out->lat : nullptr;
   sr.n_voxel = N;
-  sr.live = d->n_steps > 0 ? 1 : 0;
+  d0.live = d->n_steps > 0 ? 1 : 0;
+  sr.dyn[0] = d0;  // both copies: the slot's first step may run at either launch parity
+  sr.dyn[1] = d0;
   sr.model = d->model;
   sr.nspm = d->steps_per_sample;
   sr.n_samples = d->n_samples;
@@ -727,26 +738,30 @@ int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream) {
   if (a->brick_high == 0) return PHS_OK;
   hipStream_t s = (hipStream_t)stream;
   const int64_t grid = grid_for(a);
-  if (!a->gexec || a->g_steps != n_steps || a->g_grid != grid || a->g_high != a->brick_high ||
-      a->g_sweeps != a->sweeps_eff || a->g_lean != (int32_t)a->lean) {
-    if (a->gexec) hipGraphExecDestroy(a->gexec);
-    if (a->graph) hipGraphDestroy(a->graph);
-    a->gexec = nullptr;
-    a->graph = nullptr;
+  const int32_t par0 = a->par;
+  phs_arena::Graph& G = a->g[par0];
+  if (!G.exec || G.steps != n_steps || G.grid != grid || G.high != a->brick_high ||
+      G.sweeps != a->sweeps_eff || G.lean != (int32_t)a->lean) {
+    if (G.exec) hipGraphExecDestroy(G.exec);
+    if (G.graph) hipGraphDestroy(G.graph);
+    G.exec = nullptr;
+    G.graph = nullptr;
     if (!a->cap_stream) PHS_HIP_CHECK(hipStreamCreateWithFlags(&a->cap_stream, hipStreamNonBlocking));
     PHS_HIP_CHECK(hipStreamBeginCapture(a->cap_stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < n_steps; ++i) launch_step(a, a->cap_stream);
+    for (int i = 0; i < n_steps; ++i) launch_step(a, a->cap_stream);  // flips a->par
+    a->par = par0;  // nothing ran yet: the replay below advances it
     hipGraph_t g = nullptr;
     PHS_HIP_CHECK(hipStreamEndCapture(a->cap_stream, &g));
-    a->graph = g;
-    PHS_HIP_CHECK(hipGraphInstantiate(&a->gexec, g, nullptr, nullptr, 0));
-    a->g_steps = n_steps;
-    a->g_grid = grid;
-    a->g_high = a->brick_high;
-    a->g_sweeps = a->sweeps_eff;
-    a->g_lean = a->lean ? 1 : 0;
+    G.graph = g;
+    PHS_HIP_CHECK(hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0));
+    G.steps = n_steps;
+    G.grid = grid;
+    G.high = a->brick_high;
+    G.sweeps = a->sweeps_eff;
+    G.lean = a->lean ? 1 : 0;
   }
-  PHS_HIP_CHECK(hipGraphLaunch(a->gexec, s));
+  PHS_HIP_CHECK(hipGraphLaunch(G.exec, s));
+  a->par ^= n_steps & 1;
   for (auto& h : a->sh)
     if (h.used) h.steps_done = std::min<int64_t>(h.n_steps, h.steps_done + n_steps);
   return PHS_OK;
@@ -847,8 +862,10 @@ int phs_arena_destroy(phs_arena* a) {
   hipDeviceSynchronize();
   for (auto& h : a->sh) free_slot_host(h);
   reap(a, true);
-  if (a->gexec) hipGraphExecDestroy(a->gexec);
-  if (a->graph) hipGraphDestroy(a->graph);
+  for (auto& G : a->g) {
+    if (G.exec) hipGraphExecDestroy(G.exec);
+    if (G.graph) hipGraphDestroy(G.graph);
+  }
   if (a->cap_stream) hipStreamDestroy(a->cap_stream);
   hipFree(a->bricks);
   hipFree(a->slots);

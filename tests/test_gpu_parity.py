@@ -243,6 +243,38 @@ def test_graph_recaptured_when_bricks_added_within_grid(cuda, bpw):
     assert np.array_equal(la.cpu().numpy(), lm.cpu().numpy())
 
 
+def test_odd_graph_lengths_and_steps_past_the_end_equal_eager(cuda):
+    """The per-step slot values alternate between two copies with the launch parity.  Graph
+    replays of odd lengths (both starting parities), eager steps in between, a simulation
+    admitted at an odd parity, and steps run after both simulations have finished give the
+    same bits as plain eager stepping, and a finished simulation stays finished."""
+    from pyheartsim_amd import runtime
+    pa, pb = _prep(small_sheet(n=16, n_steps=120)), _prep(small_sheet(n=12, n_steps=90))
+    ref = {}
+    for name, p in (("a", pa), ("b", pb)):
+        alone = runtime.Arena(p.n_voxel, 1, runtime.SolverConfig())
+        s = alone.add(p)
+        alone.step(p.n_steps)
+        u, _, lat = alone.outputs(s)
+        ref[name] = (u.cpu().numpy(), lat.cpu().numpy())
+    mixed = runtime.Arena(pa.n_voxel + pb.n_voxel, 2, runtime.SolverConfig())
+    sa = mixed.add(pa)
+    mixed.step(7, graph=True)
+    mixed.step(3)
+    sb = mixed.add(pb)  # admitted after 10 steps, then 7-step replays from the other parity
+    for n, graph in ((7, True), (13, True), (7, True), (1, False), (7, True), (13, True)):
+        mixed.step(n, graph=graph)
+    while mixed.steps_done(sa) < pa.n_steps or mixed.steps_done(sb) < pb.n_steps:
+        mixed.step(7, graph=True)
+    mixed.step(13, graph=True)  # past the end of both
+    mixed.step(2)
+    for name, s in (("a", sa), ("b", sb)):
+        u, _, lat = mixed.outputs(s)
+        print(f"sim {name}: samples {u.shape}", flush=True)
+        assert np.array_equal(ref[name][0], u.cpu().numpy())
+        assert np.array_equal(ref[name][1], lat.cpu().numpy())
+
+
 def _hetero_c(w, seed=11):
     """Per-voxel conductivity (P_2d column 20, as the reference builds it from c_voxel) varying
     smoothly in [0.6, 1.4] x the configured value, with one region at 0 (scar by conductivity)."""
