@@ -470,6 +470,8 @@ struct phs_egm {
   unsigned short* Wt = nullptr;  // BF16 hi/lo planes, tiles [Kpad / 64][2][Epad][64]
   float* ws = nullptr;
   size_t ws_bytes = 0;
+  int resident[2] = {0, 0};  // workgroups of egm_gemm_lds (without / with lo) the device
+                             // holds at once, 0 = not yet queried
 };
 
 extern "C" {
@@ -560,8 +562,30 @@ int phs_egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, flo
   const int mt = (int)((T + tm - 1) / tm);
   const int nt = g->Epad / te;
   const int ksteps = (g->N + kstep - 1) / kstep;
-  // split K so that about 1024 workgroups (4 per CU) cover the device; chunks are whole K stages
-  int splits = std::max(1, std::min(ksteps, (1024 + mt * nt - 1) / (mt * nt)));
+  // Split K so that the workgroups fill the device once: as many as it holds at once (the
+  // occupancy of the kernel times the CU count).  At 64 electrodes (2 per CU) this is 5 %
+  // faster than 4 per CU, whose second round is partly a tail; at 256 electrodes (1 per CU)
+  // the count makes no measurable difference.  Chunks are whole K stages.
+  int target = 1024;
+  if (lds) {
+    const bool lo = precision == 0;
+    int& res = g->resident[lo ? 1 : 0];
+    if (res == 0) {
+      const void* kern = g->ce == 4 ? (lo ? (const void*)egm_gemm_lds<rw_for(4), 4, true>
+                                          : (const void*)egm_gemm_lds<rw_for(4), 4, false>)
+                         : g->ce == 2 ? (lo ? (const void*)egm_gemm_lds<rw_for(2), 2, true>
+                                            : (const void*)egm_gemm_lds<rw_for(2), 2, false>)
+                                      : (lo ? (const void*)egm_gemm_lds<rw_for(1), 1, true>
+                                            : (const void*)egm_gemm_lds<rw_for(1), 1, false>);
+      int per_cu = 0, dev = 0, cus = 0;
+      PHS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kGemmThreads, 0));
+      PHS_HIP_CHECK(hipGetDevice(&dev));
+      PHS_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      res = std::max(1, per_cu) * std::max(1, cus);
+    }
+    target = res;
+  }
+  int splits = std::max(1, std::min(ksteps, (target + mt * nt - 1) / (mt * nt)));
   const int k_chunk = (ksteps + splits - 1) / splits * kstep;
   splits = (g->N + k_chunk - 1) / k_chunk;
   const size_t need = sizeof(float) * (size_t)splits * T * g->Epad;
