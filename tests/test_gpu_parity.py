@@ -243,13 +243,17 @@ def test_graph_recaptured_when_bricks_added_within_grid(cuda, bpw):
     assert np.array_equal(la.cpu().numpy(), lm.cpu().numpy())
 
 
-def test_odd_graph_lengths_and_steps_past_the_end_equal_eager(cuda):
+@pytest.mark.parametrize("scar", [False, True])
+def test_odd_graph_lengths_and_steps_past_the_end_equal_eager(cuda, scar):
     """The per-step slot values alternate between two copies with the launch parity.  Graph
     replays of odd lengths (both starting parities), eager steps in between, a simulation
     admitted at an odd parity, and steps run after both simulations have finished give the
-    same bits as plain eager stepping, and a finished simulation stays finished."""
+    same bits as plain eager stepping, and a finished simulation stays finished.  With a
+    scarred second simulation the arena runs the general kernels, otherwise the lean ones."""
     from pyheartsim_amd import runtime
-    pa, pb = _prep(small_sheet(n=16, n_steps=120)), _prep(small_sheet(n=12, n_steps=90))
+    pa = _prep(small_sheet(n=16, n_steps=120))
+    pb = (_prep(small_shell(t_final=6.0, seed=5, semi_axes=(7.0, 6.0, 5.0), n_scar=1)) if scar
+          else _prep(small_sheet(n=12, n_steps=90)))
     ref = {}
     for name, p in (("a", pa), ("b", pb)):
         alone = runtime.Arena(p.n_voxel, 1, runtime.SolverConfig())
