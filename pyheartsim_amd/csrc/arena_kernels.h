@@ -277,11 +277,14 @@ struct Stencil {
   }
 };
 
+// Kernel flavours.  The lean ones run when every simulation in the arena runs the launched
+// sweep count and none has heterogeneous alpha: they drop the per-voxel alpha and the
+// identity-sweep logic (the scalar and vector work they cost is a measurable share of a sweep,
+// which is bound by instruction issue as much as by memory).  FLAV_LEAN also has no scar or
+// temporary-block voxels to test; FLAV_LEAN_FLAGS keeps that test.
+enum { FLAV_GENERAL = 0, FLAV_LEAN_FLAGS = 1, FLAV_LEAN = 2 };
+
 // Wave-uniform slot values a stencil kernel needs, loaded together.
-// LEAN kernels run when no simulation in the arena has scar, temporary-block or
-// heterogeneous-alpha voxels and every one runs the launched sweep count; they drop the
-// per-voxel alpha and the identity-sweep logic (the scalar and vector work they cost is a
-// measurable share of a sweep, which is bound by instruction issue as much as by memory).
 // Per-simulation sweep counts: a solve of K launched sweeps carries a simulation whose own
 // count is nsw < K by identity sweeps (x_{k+1} = x_k) from launch nsw on, so it computes
 // exactly what it computes alone.  Identity launches other than the last one only move the
@@ -293,10 +296,10 @@ struct SlotView {
   bool ident, skip;
   float alpha, inv_theta, ca, cb, cg;
   double alpha64;
-  template <bool LEAN = false>
+  template <int FLAV = FLAV_GENERAL>
   __device__ __forceinline__ void load(const SlotRec& S, int par, int k, int K = 0,
                                        bool last = true) {
-    if (LEAN) {
+    if (FLAV != FLAV_GENERAL) {
       ident = skip = false;
     } else {
       const int nsw = S.nsw;
@@ -322,21 +325,21 @@ struct SlotView {
 struct AlphaIn {
   float al;
   uint8_t f;
-  template <bool LEAN>
+  template <int FLAV>
   __device__ __forceinline__ void load(const ArenaDev& A, const SlotView& V, int idx) {
     al = 0.0f;
     f = 0;
-    if (LEAN) return;
-    if (V.hetero) al = ld(A.ALPHA, idx);
+    if (FLAV == FLAV_LEAN) return;
+    if (FLAV == FLAV_GENERAL && V.hetero) al = ld(A.ALPHA, idx);
     if (V.needs_vf) f = ld(A.VF, idx);
   }
 };
 
-template <typename T, bool LEAN = false>
+template <typename T, int FLAV = FLAV_GENERAL>
 __device__ __forceinline__ T alpha_at(const SlotView& V, const AlphaIn& in) {
   T a = sizeof(T) == 8 ? (T)V.alpha64 : (T)V.alpha;
-  if (LEAN) return a;
-  if (V.hetero) a = (T)in.al;
+  if (FLAV == FLAV_LEAN) return a;
+  if (FLAV == FLAV_GENERAL && V.hetero) a = (T)in.al;
   if (V.needs_vf) {
     const uint8_t f = in.f;
     if ((f & PHS_VF_SCAR) || ((V.cur_sched & PHS_SCHED_TEMP_BLOCK) && (f & PHS_VF_TEMP_BLOCK)))
@@ -495,7 +498,7 @@ __device__ __forceinline__ void advance_slot(SlotRec& S, int par) {
 // Right-hand side of the correction solve A e = r, x = u* + g + e, with the warm start g:
 //   r = dt L u* - A g = 2 alpha sum (u*_m - u*_n) - (g_n - alpha sum (g_m - g_n)),
 // formed in FP64 from FP32 u* and g.
-template <int BX, int BY, int BZ, int NB, int GUESS, bool LEAN>
+template <int BX, int BY, int BZ, int NB, int GUESS, int FLAV>
 __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
   const int b0 = first_brick<NB>(xcd_of(A));
   const int lane = lane_id();
@@ -516,7 +519,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
     }
     if (on[j]) {
       st[j].init(br);
-      sv[j].template load<LEAN>(A.slots[br.slot], par_of(A), 0);
+      sv[j].template load<FLAV>(A.slots[br.slot], par_of(A), 0);
       un[j] = ld(A.U, st[j].idx);
       st[j].gather(A.U, gu[j]);
       if (GUESS >= 1) {
@@ -524,7 +527,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
         gn0[j] = ld(G, st[j].idx);
         st[j].gather(G, gg[j]);
       }
-      ain[j].template load<LEAN>(A, sv[j], st[j].idx);
+      ain[j].template load<FLAV>(A, sv[j], st[j].idx);
     }
   }
 #pragma unroll
@@ -539,7 +542,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
       gn = gv;
     }
     if (!st[j].valid || !sv[j].live) continue;
-    const double a = alpha_at<double, LEAN>(sv[j], ain[j]);
+    const double a = alpha_at<double, FLAV>(sv[j], ain[j]);
     stv(A.R0, st[j].idx, (float)(2.0 * a * s - (gn - a * sg)));
   }
   // The first workgroup's first wavefront also stages every slot's next-step values.  This
@@ -586,7 +589,7 @@ __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotFinish
 // 2: prev[].  GUESS: warm start of the correction (see guess_array); the final modes add it
 // back and keep the step's total correction in D1 (and, with guess 2, the next step's
 // extrapolated warm start in D2).
-template <int BX, int BY, int BZ, int MODE, int NB, int GUESS, bool FIRST, bool LEAN>
+template <int BX, int BY, int BZ, int MODE, int NB, int GUESS, bool FIRST, int FLAV>
 __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, const float* cur,
                                                   const float* prev, int prev_kind, float* out) {
   const int b0 = first_brick<NB>(xcd_of(A));
@@ -619,13 +622,13 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       // the loads are issued before the skip test so that they depend on the brick record
       // only; a skipped wave drops them
       st[j].init(br);
-      sv[j].template load<LEAN>(A.slots[br.slot], par_of(A), k, K, MODE != MODE_MID);
+      sv[j].template load<FLAV>(A.slots[br.slot], par_of(A), k, K, MODE != MODE_MID);
       const int i = st[j].idx;
       dn[j] = ld(src, i);
       r0[j] = ld(A.R0, i);
       dp[j] = prev_kind == 2 ? ld(prev, i) : 0.0f;
       st[j].gather(src, g6[j]);
-      ain[j].template load<LEAN>(A, sv[j], i);
+      ain[j].template load<FLAV>(A, sv[j], i);
       if (MODE == MODE_DACC_ADD || MODE == MODE_DACC_ADD_X) pd[j] = A.DACC[i];
       if (MODE == MODE_DACC_SET) pd[j] = GUESS >= 1 ? (double)guess_array<GUESS>(A)[i] : 0.0;
       if (kFinish) {
@@ -647,7 +650,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
     const float s = st[j].template sum<float>(d, g6[j], sc, lane);
     if (!st[j].valid || !sv[j].live) continue;
     const int i = st[j].idx;
-    const float a = alpha_at<float, LEAN>(sv[j], ain[j]);
+    const float a = alpha_at<float, FLAV>(sv[j], ain[j]);
     const float Ad = d - a * s;
     const float dpv = prev_kind == 1 ? r0[j] * inv_theta : dp[j];
     const float dnew = sv[j].ident ? d : sv[j].ca * d + sv[j].cb * dpv + sv[j].cg * (r0[j] - Ad);
@@ -692,7 +695,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(ArenaDev A) {
   SlotView sv;
   sv.load(A.slots[A.bricks[b].slot], par_of(A), 0);
   AlphaIn ain;
-  ain.load<false>(A, sv, st.idx);
+  ain.load<FLAV_GENERAL>(A, sv, st.idx);
   const float un = A.U[st.idx];
   const double dn = A.DACC[st.idx];
   float gu[6];

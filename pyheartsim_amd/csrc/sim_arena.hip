@@ -123,11 +123,11 @@ struct phs_arena {
     int64_t grid = -1;
     int64_t high = -1;  // brick_high captured in the kernel arguments
     int32_t sweeps = -1;
-    int32_t lean = -1;
+    int32_t flav = -1;
   } g[2];
   int32_t par = 0;         // launch parity of the next step (flips with every launched step)
   int32_t sweeps_eff = 2;  // sweeps of the first CN solve in the launched sequence
-  bool lean = false;       // no simulation needs per-voxel alpha or identity sweeps
+  int32_t flav = 0;        // kernel flavour (FLAV_*) the resident simulations allow
 };
 
 namespace {
@@ -172,47 +172,48 @@ int grid_for(const phs_arena* a) {
 }
 
 // Launches of one time step for brick shape (BX, BY, BZ) and NB bricks per wavefront.
-// LEAN kernels (one brick per wavefront only) serve arenas whose simulations need neither
-// per-voxel alpha nor identity sweeps.
+// The lean kernel flavours (one brick per wavefront only) serve arenas whose simulations need
+// neither per-voxel alpha nor identity sweeps.
 template <int BX, int BY, int BZ, int NB, int GUESS>
 struct Step {
-  template <int MODE, bool LEAN>
+  template <int MODE, int FLAV>
   static void sweep_m(int grid, hipStream_t s, const ArenaDev& A, int k, int K, const float* cur,
                       const float* prev, int prev_kind, float* out) {
     if (k == 1)
-      k_sweep<BX, BY, BZ, MODE, NB, GUESS, true, LEAN><<<grid, kBlock, 0, s>>>(A, k, K, cur, prev,
+      k_sweep<BX, BY, BZ, MODE, NB, GUESS, true, FLAV><<<grid, kBlock, 0, s>>>(A, k, K, cur, prev,
                                                                                prev_kind, out);
     else
-      k_sweep<BX, BY, BZ, MODE, NB, GUESS, false, LEAN><<<grid, kBlock, 0, s>>>(A, k, K, cur, prev,
+      k_sweep<BX, BY, BZ, MODE, NB, GUESS, false, FLAV><<<grid, kBlock, 0, s>>>(A, k, K, cur, prev,
                                                                                 prev_kind, out);
   }
-  template <bool LEAN>
+  template <int FLAV>
   static void sweep(int mode, int grid, hipStream_t s, const ArenaDev& A, int k, int K,
                     const float* cur, const float* prev, int prev_kind, float* out) {
     switch (mode) {
-      case MODE_MID: sweep_m<MODE_MID, LEAN>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
-      case MODE_FINAL_X: sweep_m<MODE_FINAL_X, LEAN>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
-      case MODE_DACC_SET: sweep_m<MODE_DACC_SET, LEAN>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
-      case MODE_DACC_ADD: sweep_m<MODE_DACC_ADD, LEAN>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
-      default: sweep_m<MODE_DACC_ADD_X, LEAN>(grid, s, A, k, K, cur, prev, prev_kind, out);
+      case MODE_MID: sweep_m<MODE_MID, FLAV>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
+      case MODE_FINAL_X: sweep_m<MODE_FINAL_X, FLAV>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
+      case MODE_DACC_SET: sweep_m<MODE_DACC_SET, FLAV>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
+      case MODE_DACC_ADD: sweep_m<MODE_DACC_ADD, FLAV>(grid, s, A, k, K, cur, prev, prev_kind, out); break;
+      default: sweep_m<MODE_DACC_ADD_X, FLAV>(grid, s, A, k, K, cur, prev, prev_kind, out);
     }
   }
 
   static void run(phs_arena* a, hipStream_t s) {
     if constexpr (NB == 1) {
-      if (a->lean) return run_t<true>(a, s);
+      if (a->flav == FLAV_LEAN) return run_t<FLAV_LEAN>(a, s);
+      if (a->flav == FLAV_LEAN_FLAGS) return run_t<FLAV_LEAN_FLAGS>(a, s);
     }
-    run_t<false>(a, s);
+    run_t<FLAV_GENERAL>(a, s);
   }
 
-  template <bool LEAN>
+  template <int FLAV>
   static void run_t(phs_arena* a, hipStream_t s) {
     const ArenaDev A = dev_view(a);
     const int grid = grid_for(a);
     const int grid1 = grid_for_nb1(a);
     const phs_arena_config& c = a->cfg;
     // the step's reaction already ran in the previous step's last sweep (or at add)
-    k_rhs<BX, BY, BZ, NB, GUESS, LEAN><<<grid, kBlock, 0, s>>>(A);
+    k_rhs<BX, BY, BZ, NB, GUESS, FLAV><<<grid, kBlock, 0, s>>>(A);
     float* bufs[2] = {a->DA, a->DB};
     // One Chebyshev solve of m sweeps on the right-hand side held in R0; per_slot: each
     // simulation runs its own count (<= m) of the first solve.
@@ -222,7 +223,7 @@ struct Step {
         const float* cur = k == 1 ? nullptr : bufs[k % 2];
         const float* prev = k <= 2 ? nullptr : bufs[(k - 1) % 2];
         const int prev_kind = k == 1 ? 0 : (k == 2 ? 1 : 2);
-        sweep<LEAN>(k < m - 1 ? MODE_MID : last_mode, grid, s, A, k, K, cur, prev, prev_kind,
+        sweep<FLAV>(k < m - 1 ? MODE_MID : last_mode, grid, s, A, k, K, cur, prev, prev_kind,
                     bufs[(k + 1) % 2]);
       }
     };
@@ -242,14 +243,15 @@ struct Step {
   // Average duration (ms) of the reaction, rhs and sweep kernels over `reps` launches,
   // timed with HIP events on `s`.  The reaction writes into scratch (DA, DB) and the sweep
   // runs in MID mode, so no simulation state changes.  The kernels are the ones a step of
-  // the arena launches (LEAN or not).
+  // the arena launches (of its flavour).
   static hipError_t profile(phs_arena* a, int reps, double* ms, hipStream_t s) {
     if constexpr (NB == 1) {
-      if (a->lean) return profile_t<true>(a, reps, ms, s);
+      if (a->flav == FLAV_LEAN) return profile_t<FLAV_LEAN>(a, reps, ms, s);
+      if (a->flav == FLAV_LEAN_FLAGS) return profile_t<FLAV_LEAN_FLAGS>(a, reps, ms, s);
     }
-    return profile_t<false>(a, reps, ms, s);
+    return profile_t<FLAV_GENERAL>(a, reps, ms, s);
   }
-  template <bool LEAN>
+  template <int FLAV>
   static hipError_t profile_t(phs_arena* a, int reps, double* ms, hipStream_t s) {
     const ArenaDev A = dev_view(a);
     const int grid = grid_for(a);
@@ -263,10 +265,10 @@ struct Step {
     for (int i = 0; i < reps; ++i)
       k_reaction<NB><<<grid, kBlock, 0, s>>>(R, 0, (int)a->brick_high);
     hipEventRecord(ev[1], s);
-    for (int i = 0; i < reps; ++i) k_rhs<BX, BY, BZ, NB, GUESS, LEAN><<<grid, kBlock, 0, s>>>(A);
+    for (int i = 0; i < reps; ++i) k_rhs<BX, BY, BZ, NB, GUESS, FLAV><<<grid, kBlock, 0, s>>>(A);
     hipEventRecord(ev[2], s);
     for (int i = 0; i < reps; ++i)
-      k_sweep<BX, BY, BZ, MODE_MID, NB, GUESS, false, LEAN><<<grid, kBlock, 0, s>>>(
+      k_sweep<BX, BY, BZ, MODE_MID, NB, GUESS, false, FLAV><<<grid, kBlock, 0, s>>>(
           A, 3, 0, a->DA, a->DB, 2, a->DB);
     hipEventRecord(ev[3], s);
     hipError_t r = hipEventSynchronize(ev[3]);
@@ -388,10 +390,13 @@ void recompute_high(phs_arena* a) {
     }
   a->brick_high = hi;
   a->sweeps_eff = a->cfg.sweeps > 0 ? a->cfg.sweeps : sw;
-  bool lean = true;
+  bool lean = true, flags = false;
   for (auto& h : a->sh)
-    if (h.used && (h.needs_vf || h.hetero || h.need_sweeps != a->sweeps_eff)) lean = false;
-  a->lean = lean;
+    if (h.used) {
+      if (h.hetero || h.need_sweeps != a->sweeps_eff) lean = false;
+      flags = flags || h.needs_vf;
+    }
+  a->flav = !lean ? FLAV_GENERAL : (flags ? FLAV_LEAN_FLAGS : FLAV_LEAN);
 }
 
 }  // namespace
@@ -741,7 +746,7 @@ int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream) {
   const int32_t par0 = a->par;
   phs_arena::Graph& G = a->g[par0];
   if (!G.exec || G.steps != n_steps || G.grid != grid || G.high != a->brick_high ||
-      G.sweeps != a->sweeps_eff || G.lean != (int32_t)a->lean) {
+      G.sweeps != a->sweeps_eff || G.flav != a->flav) {
     if (G.exec) hipGraphExecDestroy(G.exec);
     if (G.graph) hipGraphDestroy(G.graph);
     G.exec = nullptr;
@@ -758,7 +763,7 @@ int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream) {
     G.grid = grid;
     G.high = a->brick_high;
     G.sweeps = a->sweeps_eff;
-    G.lean = a->lean ? 1 : 0;
+    G.flav = a->flav;
   }
   PHS_HIP_CHECK(hipGraphLaunch(G.exec, s));
   a->par ^= n_steps & 1;
