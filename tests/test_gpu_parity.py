@@ -85,7 +85,9 @@ def test_fp32_arena_lat_within_one_step(cuda):
 
 def test_arena_alone_equals_mixed_and_graph(cuda):
     """A simulation gives the same bits alone (lean kernels: no scar, one sweep count) as
-    packed with a scarred one (general kernels), admitted mid-run, with graph replay."""
+    packed with a scarred one (lean kernels with the scar test), admitted mid-run, with
+    graph replay.  The general kernels' bits are checked the same way by
+    test_per_simulation_sweep_counts_mixed_spacing."""
     from pyheartsim_amd import runtime
     wa = small_shell(t_final=20.0, seed=3)
     wb = small_shell(t_final=30.0, seed=5, semi_axes=(8.0, 7.0, 6.0), n_scar=1)
