@@ -275,6 +275,24 @@ struct Stencil {
     }
     return s;
   }
+
+  // The same sum formed as (sum of the six m) - 6 v in T: one conversion and one add per
+  // direction instead of a conversion, a subtraction and an add.  For T = double and V = float
+  // the rounding is ~1e-16 of |v|, far below the FP32 rounding of the right-hand side it
+  // feeds.
+  template <typename T, typename V>
+  __device__ __forceinline__ T sum_total(V v, const V (&g)[6], V sc, int lane) const {
+    const V pv[6] = {exch<0>(v, lane), exch<1>(v, lane), exch<2>(v, lane),
+                     exch<3>(v, lane), exch<4>(v, lane), exch<5>(v, lane)};
+    T s;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      V m = lane_bit(pcr[d]) ? g[d] * sc : v;
+      m = lane_bit(pin[d]) ? pv[d] : m;
+      s = d == 0 ? (T)m : s + (T)m;
+    }
+    return __builtin_fma((T)-6, (T)v, s);
+  }
 };
 
 // Kernel flavours.  The lean ones run when every simulation in the arena runs the launched
@@ -534,11 +552,11 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
   for (int j = 0; j < NB; ++j) {
     if (!on[j]) continue;
     const float uv = st[j].valid ? un[j] : 0.0f;
-    const double s = st[j].template sum<double>(uv, gu[j], 1.0f, lane);
+    const double s = st[j].template sum_total<double>(uv, gu[j], 1.0f, lane);
     double sg = 0.0, gn = 0.0;
     if (GUESS >= 1) {
       const float gv = st[j].valid ? gn0[j] : 0.0f;
-      sg = st[j].template sum<double>(gv, gg[j], 1.0f, lane);
+      sg = st[j].template sum_total<double>(gv, gg[j], 1.0f, lane);
       gn = gv;
     }
     if (!st[j].valid || !sv[j].live) continue;
