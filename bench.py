@@ -303,6 +303,7 @@ def run_precision(args, torch, runtime):
     out["plain"] = dict(voxel_steps_per_s=plain_rate, sweeps=plain.info()[0],
                         lambda_max=plain.info()[2])
     ref = plain.lat.cpu().numpy()
+    u_plain = plain.state()[0]          # FP64 state after the last step, caller order
     del plain
     # automatic sweep count (stability + accuracy rule) without refinement, and with three
     # FP64-residual refinement passes of 12 sweeps (CN residual <= 1e-10)
@@ -319,12 +320,15 @@ def run_precision(args, torch, runtime):
         dt = time.perf_counter() - t0
         res = arena.last_residual(slot)
         _, _, lat = arena.outputs(slot)
+        u_end = arena.state(slot)[0]    # x after the simulation's last step
+        dvm = float((u_end.double() - u_plain).abs().max().item())
         rate = N * prep.n_steps / dt
         out[f"refine{refine}"] = dict(
             voxel_steps_per_s=rate, speedup_vs_plain=rate / plain_rate, sweeps=arena.sweeps(),
             refine_sweeps=rsw if refine else 0,
             stable_sweeps=runtime.stable_sweeps(lmax, args.guess, refine, rsw)[0],
-            lambda_max=lmax, cn_rel_residual=res, lat_err_steps=lat_error(lat.cpu().numpy(), ref, prep.n_steps))
+            lambda_max=lmax, cn_rel_residual=res, max_abs_dVm_final=dvm,
+            lat_err_steps=lat_error(lat.cpu().numpy(), ref, prep.n_steps))
         log(f"precision refine={refine}: {out[f'refine{refine}']}")
         arena.close()
     # electrogram: 256 electrodes x 2048 samples over all voxels, synthetic activation samples
