@@ -210,7 +210,8 @@ def run_ragged(args, torch, runtime):
         plain = runtime.PlainSim(preps[i], tol=1e-15, samples=False)
         plain.run()
         _, _, lat = arena.outputs(slots[i])
-        val.append(dict(sim=i, n_voxel=ns[i],
+        dvm = float((arena.state(slots[i])[0].double() - plain.state()[0]).abs().max().item())
+        val.append(dict(sim=i, n_voxel=ns[i], max_abs_dVm_final=dvm,
                         lat_err_steps=lat_error(lat.cpu().numpy(), plain.lat.cpu().numpy(),
                                                 preps[i].n_steps)))
         del plain
