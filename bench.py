@@ -371,8 +371,10 @@ def main():
     ap.add_argument("--sweeps", type=int, default=0,
                     help="Chebyshev sweeps per CN solve; 0 = the arena's automatic count (the "
                          "fewest that keep the step stable for the spectral bound, with margin), "
-                         "-1 = the fewest stable (shell workload: 5 at lambda_max 8.5; its "
-                         "validation meets LAT <= 1 dt but not the 1e-3 electrogram bar)")
+                         "-1 = the fewest stable for the chosen warm start (shell workload at "
+                         "lambda_max 8.5: 6 with --guess 2, the automatic count; 5 with "
+                         "--guess 1, whose validation meets LAT <= 1 dt but not the 1e-3 "
+                         "electrogram bar)")
     ap.add_argument("--guess", type=int, default=2,
                     help="warm start of the CN correction: 0 none, 1 previous step, 2 linear "
                          "extrapolation of the previous two")
