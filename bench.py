@@ -1,33 +1,40 @@
 #!/usr/bin/env python
 """Throughput of the simulation hot path on one MI355X.
 
-Workload: the Mitchell-Schaeffer left-atrium-like shell (ellipsoid 25/20/18 mm,
-3 mm wall, 4 vein holes of 6 mm, 0.4 mm voxels), dt = 0.05 ms, focal pacing at
-CL 500 ms for 2000 ms (40 000 steps), 64 basket electrodes at 1 kHz.
+Headline workload (default, `--workload batched`): the batched form.  64 seeded
+Mitchell-Schaeffer ellipsoidal shells of uneven size (semi-axes U(15, 30) mm, wall U(2, 4) mm,
+0.4 mm voxels, 0-3 scar blobs, a seeded pacing site, cycle length U(250, 600) ms), each 1000 ms
+at dt = 0.05 ms (20 000 steps) with a 64-electrode electrogram sampled at 1 kHz, FP32
+simulation + BF16 hi/lo MFMA electrogram, packed into one arena whose brick shape comes from
+the autotuner, steps replayed from hipGraphs.
 
-Legs (all on cuda:0, one process):
-  plain    plain FP64 form, one simulation (reference algorithm, original voxel
-           order, CN solve iterated to 1e-15), the denominator of the speed-up
-  single   chosen form (bricked FP32, hipGraph steps), one simulation
-  batched  chosen form, `--batch` simulations of the shell with different seeded
-           pacing sites packed into one launch sequence -> `value`
-  validate full 40 000-step runs of plain and single: LAT error (steps), max |dVm|
-  egm      BF16 hi/lo MFMA electrogram of the single run's samples vs the FP32
-           MFMA electrogram of the plain run's samples
-  serve    open-loop Poisson request stream (--serve-requests at --qps) through the
-           continuous-batching loop: p50/p99 request latency and served voxel-steps/s
-  cpu      the FP64 oracle with the reference's BiCGSTAB stop rule on a bounded
-           window of steps from a mid-activation state (host cores)
+A step is one dt of reaction + the full CN solve for every voxel of every resident
+simulation, plus the electrogram contraction of the samples that step produces (contracted
+after every graph chunk from a per-simulation sample ring).  `value` = voxel-steps/s over
+exactly --steps timed steps, inputs resident in HBM.  The same run then
 
-A step is one dt of reaction + full CN solve for every voxel of every resident
-simulation; value = voxel-steps/s over exactly --steps timed steps of the
-batched leg, inputs resident in HBM.  One JSON line goes to stdout.
+  * finishes all 64 simulations (the sustained rate over those steps is reported too) and
+    validates two of them against the plain FP64 form over the full 1000 ms: LAT error in
+    steps, and electrogram rel-L2 of the chosen chain (FP32 simulation + BF16 hi/lo MFMA)
+    against the plain chain (FP64 simulation + FP32 MFMA);
+  * times the plain FP64 form on those simulations (the denominator of the speed-up);
+  * times the 0.4 mm shell of 199 446 voxels alone and 16 copies packed;
+  * serves a Poisson request stream from the served workload's ranges (p50/p99);
+  * times the CPU baseline: the C/OpenMP FP64 restatement of legacy/compute_simulation_cpu.py
+    (A rebuilt every step, BiCGSTAB with the reference's stop rule) on one member shell from
+    a mid-activation state, on the host cores this job may use.
+
+Other workloads: `shell` (the 0.4 mm shell alone and x16, full 40 000-step validation),
+`precision` (0.2 mm shell with scar, FP64-residual refinement, 256-electrode electrogram),
+`egm` (electrogram alone), `serve` (a longer served stream).  One JSON line goes to stdout.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
+import platform
 import sys
 import time
 
@@ -37,12 +44,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BRICK_RECORD_BYTES = 192  # sizeof(BrickRec): masks, gather bases and slot of one 64-cell brick
 BF16_PEAK_TFLOPS = 2500.0
+BRICK_RECORD_BYTES = 192  # sizeof(BrickRec): masks, gather bases and slot of one 64-cell brick
+METRIC = "voxel-steps/s at LAT err<=1dt (chosen vs plain FP64); p50/p99 sim latency"
+
+PROFILES = os.path.join(ROOT, "profiles")
+PROFILE_TRAFFIC = (os.path.join(PROFILES, "arena_sweep_pmc_batched64.md"),
+                   os.path.join(PROFILES, "arena_sweep_pmc_shell_ms_x16.md"))
+PROFILE_TRAFFIC_EGM = os.path.join(PROFILES, "egm_shell_e64_t2000.md")
 
 
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "arena_sweep_pmc_shell_ms_x16.md")
-PROFILE_TRAFFIC_EGM = os.path.join(ROOT, "profiles", "egm_shell_e64_t2000.md")
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def _profile_record(path, match, key):
@@ -61,55 +74,45 @@ def _profile_record(path, match, key):
 
 
 def profiled_traffic(cnt):
-    """HBM bytes (read + write) of one middle-sweep launch, when the committed summary was
+    """HBM bytes (read + write) of one middle-sweep launch, when a committed summary was
     measured on an arena of the same voxels and bricks."""
-    return _profile_record(PROFILE_TRAFFIC, {"voxels": cnt["voxels"], "bricks": cnt["bricks"],
-                                             "record_bytes": BRICK_RECORD_BYTES},
-                           "sweep_bytes_per_launch")
+    for path in PROFILE_TRAFFIC:
+        r = _profile_record(path, {"voxels": cnt["voxels"], "bricks": cnt["bricks"],
+                                   "record_bytes": BRICK_RECORD_BYTES}, "sweep_bytes_per_launch")
+        if r is not None:
+            return r
+    return None
 
 
-def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+def host_threads() -> int:
+    """Threads this job may run on the host: OMP_NUM_THREADS when set (the share of the host
+    given to the job), else every core in the process's affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
-def build_workload(seed_sites):
-    from pyheartsim_amd import configs, geometry as geo, runtime
-    g = geo.shell((25.0, 20.0, 18.0), 3.0, 0.4, 4, 6.0, 0)
-    preps = []
-    work = None
-    for s in seed_sites:
-        w = configs.shell_ms(seed=s, geometry=g)
-        P = configs.build_P_2d(g, w["sim"], w["hm"])
-        preps.append(runtime.prepare(P.shape[0], P, g, w["sim"], w["arr"]))
-        if work is None:
-            work = w
-            work["P_2d"] = P
-    return g, work, preps
-
-
-def timed(torch, fn, steps, warmup, chunk):
-    done = 0
-    while done < warmup:
-        n = min(chunk, warmup - done)
-        fn(n)
-        done += n
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    done = 0
-    while done < steps:
-        n = min(chunk, steps - done)
-        fn(n)
-        done += n
-    torch.cuda.synchronize()
-    return time.perf_counter() - t0
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
 
 
 def lat_error(a, b, n_steps=None):
     """LAT difference (steps) of two (beats x voxels) tables, -1 = no activation.  With
     n_steps, an activation in the last step of the window that the other form lacks counts
     as a 1-step difference (the other form crosses one step later, outside the window)."""
-    a = a.astype(np.int64)
-    b = b.astype(np.int64)
+    a = np.asarray(a).astype(np.int64)
+    b = np.asarray(b).astype(np.int64)
     both = (a >= 0) & (b >= 0)
     only = (a >= 0) != (b >= 0)
     at_end = 0
@@ -125,118 +128,453 @@ def lat_error(a, b, n_steps=None):
                 compared=int(d.size))
 
 
-def cpu_baseline(work, u0, h0, step0, budget_s):
-    """C/OpenMP FP64 restatement of legacy/compute_simulation_cpu.py (reference BiCGSTAB
-    stop rule) on the host cores, from a mid-activation state, for ~budget_s seconds."""
+def step_bytes(cnt, sweeps, guess=2, samples_per_step=0.0):
+    """Algorithmic HBM bytes of one arena step: each array element read or written once by
+    each kernel, the 192-byte brick record once per kernel and brick, the flag byte of
+    scar / temporary-block simulations in every kernel that reads it, and the sample row.
+    Kernels: right-hand side (u*, warm start in; r out), first sweep (r in; d out), sweeps
+    2..m-1 (d_k, d_k-1, r in; d_k+1 out), last sweep (d_k, d_k-1, r, u*, h, flags, d_1, warm
+    start in; u*, h, d_1, warm start out, next step's reaction included)."""
+    V, B, F = cnt["voxels"], cnt["bricks"], cnt["vf_voxels"]
+    rec = B * BRICK_RECORD_BYTES
+    last = 50 if guess == 2 else (42 if guess == 1 else 34)
+    per = dict(rhs=12 * V + rec + F, first=8 * V + rec + F, mid=16 * V + rec + F,
+               last=last * V + rec)
+    total = per["rhs"] + per["first"] + (sweeps - 3) * per["mid"] + per["last"] + \
+        4 * V * samples_per_step
+    return total, per
+
+
+def mid_sweep_roofline(cnt, prof, sweeps, ms_per_step, guess=2):
+    """The roofline record of the dominant kernel (the middle Chebyshev sweep: launched
+    m - 3 times per step, the largest share of the step) from its HIP-event duration."""
+    total, per = step_bytes(cnt, sweeps, guess)
+    achieved = per["mid"] / (prof["sweep"] * 1e-3) / 1e9
+    share = (sweeps - 3) * prof["sweep"] / ms_per_step if ms_per_step > 0 else None
+    return {
+        "bound": "hbm", "kernel": "k_sweep (middle Chebyshev sweep, FP32)",
+        "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(cnt),
+        "bytes_per_launch": per["mid"], "avg_launch_ms": prof["sweep"],
+        "launches_per_step": sweeps - 3, "share_of_step": share,
+        "kernels_ms": prof,
+        "step_algorithmic_bytes": total,
+        "step_hbm_frac": total / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+
+
+# ------------------------------------------------------------------------- workload builders
+def _batched_member(kw):
+    """One member of the batched workload, built on the host (runs in a worker process)."""
+    from pyheartsim_amd import configs, runtime
+    w = configs.shell_ms(**kw)
+    g = w["geometry"]
+    P = configs.build_P_2d(g, w["sim"], w["hm"])
+    prep = runtime.prepare(P.shape[0], P, g, w["sim"], w["arr"])
+    return dict(prep=prep, voxel=g["voxel"], c_voxel=w["hm"]["c_voxel"], delta=g["Delta"],
+                electrodes=w["electrodes"], n_scar=kw["n_scar"],
+                scar_voxels=int(np.count_nonzero(w["arr"]["node_flag"] == 4)),
+                cycle_length=kw["cycle_length"])
+
+
+def _shell_member(seed):
+    """The 0.4 mm shell (25/20/18 mm, 3 mm wall, 4 vein holes) paced at seeded site `seed`."""
+    from pyheartsim_amd import configs, geometry as geo, runtime
+    g = geo.shell((25.0, 20.0, 18.0), 3.0, 0.4, 4, 6.0, 0)
+    w = configs.shell_ms(seed=seed, geometry=g)
+    P = configs.build_P_2d(g, w["sim"], w["hm"])
+    prep = runtime.prepare(P.shape[0], P, g, w["sim"], w["arr"])
+    return dict(prep=prep, voxel=g["voxel"], c_voxel=w["hm"]["c_voxel"], delta=g["Delta"],
+                electrodes=w["electrodes"])
+
+
+def build_parallel(fn, items, workers):
+    from pyheartsim_amd.serve import parallel_map
+    return parallel_map(fn, items, workers)
+
+
+# ------------------------------------------------------------------------- chosen-form chain
+class Chain:
+    """An arena of simulations with their electrograms: steps in hipGraph chunks and, after
+    each chunk, contracts the sample rows the chunk produced (read from each simulation's
+    sample ring) into that simulation's electrogram on the same stream."""
+
+    def __init__(self, torch, runtime, members, solver, ring=32, egm=True, precision="bf16x3"):
+        from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
+        self.torch = torch
+        self.members = members
+        self.ring = ring
+        self.precision = precision
+        n_vox = sum(m["prep"].n_voxel for m in members)
+        self.arena = runtime.Arena(n_vox, len(members), solver)
+        self.slots = [self.arena.add(m["prep"], samples=egm, ring=ring if egm else 0)
+                      for m in members]
+        self.lfs, self.egms, self.rows = [], [], []
+        if egm:
+            for m, s in zip(members, self.slots):
+                nb6 = m["prep"].nb
+                self.lfs.append(LeadField(m["electrodes"], m["voxel"], None, m["c_voxel"],
+                                          m["delta"], nb6, perm=self.arena.slots[s].perm))
+                self.egms.append(torch.empty((m["prep"].n_saved(), len(m["electrodes"])),
+                                             dtype=torch.float32, device="cuda"))
+                self.rows.append(0)
+        self.egm = egm
+        self.steps = 0   # steps launched (every member starts at step 0)
+
+    def produced(self, i):
+        p = self.members[i]["prep"]
+        done = min(p.n_steps, self.steps)
+        nsp = p.steps_per_sample
+        return min(p.n_saved(), (done + nsp - 1) // nsp if nsp > 0 else done)
+
+    def contract(self):
+        for i, lf in enumerate(self.lfs):
+            upto = self.produced(i)
+            su = self.arena.slots[self.slots[i]].su
+            while self.rows[i] < upto:
+                r0 = self.rows[i] % self.ring
+                n = min(upto - self.rows[i], self.ring - r0)
+                lf.run(su[r0:r0 + n], precision=self.precision, normalize=False,
+                       out=self.egms[i][self.rows[i]:self.rows[i] + n])
+                self.rows[i] += n
+
+    def advance(self, n, chunk):
+        """n steps in graph chunks of `chunk` (the remainder as eager steps), contracting
+        after each chunk."""
+        done = 0
+        while done < n:
+            c = min(chunk, n - done)
+            self.arena.step(c, graph=(c == chunk))
+            self.steps += c
+            done += c
+            if self.egm:
+                self.contract()
+
+    def warm_egm(self, rows):
+        """One contraction of `rows` rows per member into scratch, so the split-K workspaces
+        are sized before any timed region."""
+        for i, lf in enumerate(self.lfs):
+            su = self.arena.slots[self.slots[i]].su
+            scratch = self.torch.empty((rows, self.egms[i].shape[1]), dtype=self.torch.float32,
+                                       device="cuda")
+            lf.run(su[:rows], precision=self.precision, normalize=False, out=scratch)
+
+    def normalized_egm(self, i):
+        from pyheartsim_amd import _lib
+        e = self.egms[i].clone()
+        _lib.check(self.arena.lib.phs_egm_normalize(_lib.ptr(e), e.shape[0], e.shape[1], 1.0,
+                                                    _lib.stream_ptr()), "phs_egm_normalize")
+        return e
+
+    def close(self):
+        for lf in self.lfs:
+            lf.close()
+        self.arena.close()
+
+
+def timed_chain(torch, chain, steps, warmup, chunk):
+    """W untimed warm-up steps, then exactly `steps` steps timed between device
+    synchronisations; graphs of the timed chunk length are captured beforehand."""
+    if steps <= 0:
+        raise SystemExit("--steps must be > 0")
+    c = min(chunk, steps)
+    chain.arena.prepare_graph(c)
+    if warmup > 0:
+        chain.advance(warmup, min(chunk, warmup))
+        chain.arena.prepare_graph(c)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    chain.advance(steps, c)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(kw, u0, h0, step0, budget_s):
+    """C/OpenMP FP64 restatement of legacy/compute_simulation_cpu.py (A = I - dt/2 L formed
+    every step, BiCGSTAB with the reference stop rule) on the host threads this job may use,
+    from a mid-activation state of one member shell, for ~budget_s seconds."""
     from oracle import cpu_oracle
-    threads = min(16, cpu_oracle.threads_available())
-    cs = cpu_oracle.CpuSim(work["P_2d"], work["geometry"], work["sim"], work["arr"])
+    from pyheartsim_amd import configs
+    threads = host_threads()
+    w = configs.shell_ms(**kw)
+    P = configs.build_P_2d(w["geometry"], w["sim"], w["hm"])
+    cs = cpu_oracle.CpuSim(P, w["geometry"], w["sim"], w["arr"])
     cs.u[:] = u0
     cs.h[:] = h0
     cs.step = step0
-    cs.run(2, threads)                 # warm-up (first-touch of the CSR arrays)
-    # chunks of `chunk` steps until the budget is spent; each call rebuilds the CSR operator
-    # (the reference rebuilds A every step, legacy/compute_simulation_cpu.py:131-134)
-    chunk = 50
-    steps = 0
-    fails = 0
+    cs.run(2, threads)                 # warm-up (first touch of the operator arrays)
+    chunk, steps, fails = 20, 0, 0
     t0 = time.perf_counter()
     while True:
-        fails += cs.run(chunk, threads)
+        fails += cs.run(chunk, threads, rebuild=True)
         steps += chunk
         dt = time.perf_counter() - t0
         if dt >= budget_s or steps >= 20000:
             break
     n = cs.n
     return dict(value=n * steps / dt, unit="voxel-steps/s", cores=threads, kind="port",
-                sample=f"{steps} steps from step {step0} (mid-activation) of the {n}-voxel shell: "
-                       "C/OpenMP FP64 restatement of legacy/compute_simulation_cpu.py, CSR L, "
+                sample=f"{steps} steps from step {step0} (mid-activation) of one {n}-voxel member "
+                       "shell of the batched workload: C/OpenMP FP64 restatement of "
+                       "legacy/compute_simulation_cpu.py, CSR L, A = I - dt/2 L formed every step, "
                        "BiCGSTAB(atol 1e-6, rtol 1e-5, maxiter 1000) from x0 = u*",
+                cpu_model=cpu_model(), os_cpu_count=os.cpu_count(),
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+                build="gcc -O3 -march=x86-64-v2 -fopenmp -ffp-contract=off",
                 maxiter_hits=int(fails), seconds=dt)
 
 
-def fewest_stable_sweeps(runtime, preps, guess):
-    """(sweeps, lambda_max): the fewest Chebyshev sweeps whose inexact CN step grows no mode
-    (amplification <= 1 over the Gershgorin interval [1, lambda_max]) for the largest spectral
-    bound among `preps`, read from a probe arena."""
-    probe = runtime.Arena(sum(p.n_voxel for p in preps), len(preps), runtime.SolverConfig(sweeps=0))
-    lam = max(probe.bounds(probe.add(p, samples=False, max_beats=1))[1] for p in preps)
-    probe.close()
-    return runtime.stable_sweeps(lam, guess)[0], lam
-
-
-def run_ragged(args, torch, runtime):
-    """64 seeded shells of uneven size (0.4 mm, 1000 ms, 0-3 scar blobs, CL 250-600 ms) packed
-    into one arena; throughput over --steps graph-replayed steps; LAT of two of them against
-    the plain FP64 form over the full 1000 ms.  The brick shape and bricks per wavefront come
-    from the autotuner (pyheartsim_amd/autotune.py), run on every 8th shell: the two shapes of
-    best brick fill, one or two bricks per wavefront, timed on the device."""
-    from pyheartsim_amd import autotune, configs
+def validate_member(torch, runtime, chain, i, lf_cls, cpu_state_step=None):
+    """Plain FP64 run of member i over its whole duration (timed), compared with the chosen
+    form already finished in `chain`: LAT error and the electrogram chain rel-L2.  Returns
+    (record, plain rate, (u, h) at cpu_state_step or None)."""
+    m = chain.members[i]
+    prep = m["prep"]
+    plain = runtime.PlainSim(prep, tol=1e-15, samples=True, sample_h=False, sample_fp64=False)
+    state = None
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ws = configs.ragged(args.ragged_sims)
-    preps = []
-    for w in ws:
-        P = configs.build_P_2d(w["geometry"], w["sim"], w["hm"])
-        preps.append(runtime.prepare(P.shape[0], P, w["geometry"], w["sim"], w["arr"]))
-        del w["geometry"]["neighbor_id_2d"]
-    ns = [p.n_voxel for p in preps]
-    log(f"ragged: {len(preps)} sims, N {min(ns)}-{max(ns)}, total {sum(ns)}, setup "
-        f"{time.perf_counter()-t0:.1f}s")
+    if cpu_state_step:
+        plain.run(cpu_state_step)
+        torch.cuda.synchronize()
+        t_pause = time.perf_counter()
+        u, h = plain.state()
+        state = (u.cpu().numpy(), h.cpu().numpy())
+        t0 += time.perf_counter() - t_pause
+    plain.run()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    rate = prep.n_voxel * prep.n_steps / t_plain
+    lat_c = chain.arena.lat(chain.slots[i]).cpu().numpy()
+    le = lat_error(lat_c, plain.lat.cpu().numpy(), prep.n_steps)
+    lf = lf_cls(m["electrodes"], m["voxel"], None, m["c_voxel"], m["delta"], prep.nb)
+    e_ref = lf.run(plain.su, precision="fp32")
+    e_c = chain.normalized_egm(i)
+    torch.cuda.synchronize()
+    rel = float(((e_c - e_ref).norm() / e_ref.norm()).item())
+    dvm = float((chain.arena.state(chain.slots[i])[0].double() - plain.state()[0]).abs().max()
+                .item())
+    lf.close()
+    rec = dict(member=i, n_voxel=prep.n_voxel, steps=prep.n_steps, n_scar=m.get("n_scar"),
+               scar_voxels=m.get("scar_voxels"), cycle_length_ms=m.get("cycle_length"),
+               lat_err_steps=le, egm_rel_l2=rel, max_abs_dVm_final=dvm,
+               plain_voxel_steps_per_s=rate, plain_sweeps=plain.info()[0])
+    del plain
+    return rec, rate, state
+
+
+def shell_legs(torch, runtime, solver, workers, steps=400, warmup=100, chunk=50):
+    """The 0.4 mm shell (199 446 voxels): plain FP64, chosen form alone, and 16 copies packed
+    (different seeded pacing sites), each timed over `steps` steps without electrograms."""
+    members = build_parallel(_shell_member, list(range(16)), workers)
+    prep = members[0]["prep"]
+    N = prep.n_voxel
+    out = dict(n_voxel=N)
+    plain = runtime.PlainSim(prep, tol=1e-15, samples=False)
+    plain.run(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain.run(steps)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    out["plain_fp64"] = dict(voxel_steps_per_s=N * steps / t, ms_per_step=1e3 * t / steps,
+                             sweeps=plain.info()[0])
+    del plain
+    for name, ms in (("alone", members[:1]), ("x16", members)):
+        ch = Chain(torch, runtime, ms, solver, egm=False)
+        t = timed_chain(torch, ch, steps, warmup, chunk)
+        v = ch.arena.counts()["voxels"]
+        out[name] = dict(voxel_steps_per_s=v * steps / t, ms_per_step=1e3 * t / steps,
+                         sweeps=ch.arena.sweeps(), voxels=v)
+        if name == "alone":
+            total, _ = step_bytes(ch.arena.counts(), ch.arena.sweeps(), solver.guess)
+            out[name]["step_hbm_frac"] = total / (t / steps) / 1e9 / HBM_PEAK_GBS
+        ch.close()
+        torch.cuda.empty_cache()
+    return out
+
+
+def serve_leg(torch, runtime, solver, reqs, qps, max_voxels=60_000_000):
+    from pyheartsim_amd import serve
+    srv = serve.Server(max_voxels=min(max_voxels, sum(r.prep.n_voxel for r in reqs)),
+                       max_sims=64, hbm_budget_bytes=200e9, solver=solver, ring=256,
+                       chunk_steps=100)
+    st = srv.run(reqs, progress=lambda r: log(f"  served rid={r['rid']} N={r['n_voxel']} "
+                                               f"steps={r['steps']} latency={r['latency']:.2f}s"))
+    srv.arena.close()
+    ns = [r.prep.n_voxel for r in reqs]
+    return dict(qps=qps, requests=st.completed, p50_s=st.p50, p99_s=st.p99,
+                voxel_steps_per_s=st.throughput, wall_s=st.wall,
+                n_voxel_range=[min(ns), max(ns)],
+                duration_ms_range=[min(r.duration_ms for r in reqs),
+                                   max(r.duration_ms for r in reqs)],
+                workload="Poisson arrivals; shells of 2e4-2e6 voxels (log-uniform) at 0.4 mm, "
+                         "MS/AP 50/50, 500-5000 ms, 32-256 electrodes")
+
+
+# ------------------------------------------------------------------------- workloads
+def run_batched(args, torch, runtime, pre):
+    """The headline: 64 uneven shells with electrograms (see the module docstring)."""
+    from pyheartsim_amd import autotune
+    from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
+    members, kws, reqs = pre["members"], pre["kws"], pre.get("requests")
+    ns = [m["prep"].n_voxel for m in members]
     base = runtime.SolverConfig(sweeps=max(args.sweeps, 0), guess=args.guess)
     t0 = time.perf_counter()
-    solver, tune_table = autotune.tune(preps[::8], base, steps=200, warmup=50,
-                                       graph_steps=args.graph_steps)
+    solver, tune_table = autotune.tune([m["prep"] for m in members[::8]], base, steps=200,
+                                       warmup=50, graph_steps=50)
     tune = dict(brick=list(solver.brick), bricks_per_wave=solver.bricks_per_wave,
                 seconds=time.perf_counter() - t0,
                 candidates=[dict(brick=r["brick"], bricks_per_wave=r["bricks_per_wave"],
                                  fill=r["fill"], voxel_steps_per_s=r["voxel_steps_per_s"])
                             for r in tune_table])
-    log(f"ragged autotune: {tune}")
-    arena = runtime.Arena(sum(ns), len(preps), solver)
-    slots = [arena.add(p, samples=False) for p in preps]
+    log(f"autotune: {tune}")
+    torch.cuda.empty_cache()
+    t0 = time.perf_counter()
+    chain = Chain(torch, runtime, members, solver, ring=args.ring)
+    cnt = chain.arena.counts()
+    sweeps = chain.arena.sweeps()
+    nspm = members[0]["prep"].steps_per_sample
     G = args.graph_steps
-    ks = (args.steps // G) * G
-    t = timed(torch, lambda n: arena.step(n, graph=True), ks, (args.warmup // G) * G, G)
-    value = sum(ns) * ks / t
-    prof = arena.profile(20)
-    cnt = arena.counts()
-    log(f"ragged: {value:.4g} voxel-steps/s, {1e3*t/ks:.4f} ms/step, sweeps {arena.sweeps()}, "
-        f"kernels {prof}, {cnt}")
-    # validation: run two of them to the end in the arena, compare with plain FP64
-    while min(arena.steps_done(s) for s in slots) < preps[0].n_steps:
-        arena.step(G, graph=True)
-    val = []
-    for i in (0, len(preps) // 2):
-        plain = runtime.PlainSim(preps[i], tol=1e-15, samples=False)
-        plain.run()
-        _, _, lat = arena.outputs(slots[i])
-        dvm = float((arena.state(slots[i])[0].double() - plain.state()[0]).abs().max().item())
-        val.append(dict(sim=i, n_voxel=ns[i], max_abs_dVm_final=dvm,
-                        lat_err_steps=lat_error(lat.cpu().numpy(), plain.lat.cpu().numpy(),
-                                                preps[i].n_steps)))
-        del plain
-    log(f"ragged validation: {val}")
-    bytes_sweep = cnt["voxels"] * 16 + cnt["vf_voxels"] + cnt["bricks"] * BRICK_RECORD_BYTES
-    return dict(workload="ragged", sims=len(preps), voxels=sum(ns), voxel_steps_per_s=value,
-                sweeps=arena.sweeps(), autotune=tune,
-                ms_per_step=1e3 * t / ks, steps=ks, kernels_ms=prof,
-                sweep_gbs=bytes_sweep / (prof["sweep"] * 1e-3) / 1e9, validate=val)
+    assert math.ceil(G / nspm) + 1 <= args.ring, "sample ring too short for the graph chunk"
+    chain.warm_egm(math.ceil(G / nspm) + 1)
+    log(f"arena: {cnt}, sweeps {sweeps}, added in {time.perf_counter()-t0:.1f}s")
+    K, W = args.steps, args.warmup
+    t = timed_chain(torch, chain, K, W, G)
+    value = cnt["voxels"] * K / t
+    ms_step = 1e3 * t / K
+    log(f"batched: {value:.4g} voxel-steps/s, {ms_step:.4f} ms/step over {K} steps")
+    # sustained rate: the rest of the 1000 ms of every member, electrograms included
+    n_total = members[0]["prep"].n_steps
+    rest = n_total - chain.steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    chain.advance(rest, G)
+    torch.cuda.synchronize()
+    t_rest = time.perf_counter() - t0
+    sustained = dict(voxel_steps_per_s=cnt["voxels"] * rest / t_rest,
+                     ms_per_step=1e3 * t_rest / rest, steps=rest)
+    log(f"sustained: {sustained}")
+    prof = chain.arena.profile(50)
+    roof = mid_sweep_roofline(cnt, prof, sweeps, ms_step, args.guess)
+    roof["sustained_step_hbm_frac"] = (roof["step_algorithmic_bytes"] /
+                                       (sustained["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+    log(f"kernels: {prof}")
+    # validation against the plain FP64 form: the largest member and the most scarred one
+    big = int(np.argmax(ns))
+    scarred = max((i for i in range(len(members)) if i != big),
+                  key=lambda i: members[i]["scar_voxels"])
+    val, plain_rates, cpu_state = [], [], None
+    for i in (big, scarred):
+        rec, rate, st = validate_member(torch, runtime, chain, i, LeadField,
+                                        cpu_state_step=args.cpu_from_step if i == big else None)
+        val.append(rec)
+        plain_rates.append(rate)
+        if st is not None:
+            cpu_state = st
+        log(f"validate: {rec}")
+    egm_flop = sum(2.0 * len(m["electrodes"]) * m["prep"].n_saved() * m["prep"].n_voxel
+                   for m in members)
+    chain.close()
+    del chain
+    torch.cuda.empty_cache()
+    plain_rate = sum(v["plain_voxel_steps_per_s"] * v["n_voxel"] for v in val) / \
+        sum(v["n_voxel"] for v in val)
+
+    shell = None
+    if not args.only_headline:
+        shell = shell_legs(torch, runtime, solver, args.workers)
+        log(f"shell: {shell}")
+    serve_res = None
+    if reqs:
+        serve_res = serve_leg(torch, runtime, solver, reqs, args.qps)
+        log(f"serve: {serve_res}")
+    cpu = None
+    if args.cpu_seconds > 0 and cpu_state is not None:
+        cpu = cpu_baseline(kws[big], cpu_state[0], cpu_state[1], args.cpu_from_step,
+                           args.cpu_seconds)
+        log(f"cpu: {cpu}")
+    return {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-steps/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched: {len(members)} seeded MS ellipsoidal shells (semi-axes "
+                        f"15-30 mm, wall 2-4 mm, 0.4 mm voxels, 0-3 scar blobs, CL 250-600 ms), "
+                        f"N {min(ns)}-{max(ns)} (total {sum(ns)}), 1000 ms at dt 0.05 ms, "
+                        f"64-electrode BF16 hi/lo electrograms at 1 kHz, one arena, hipGraph steps",
+            "sims": len(members), "voxels": sum(ns), "n_voxel_range": [min(ns), max(ns)],
+            "steps_per_sim": n_total, "sweeps": sweeps, "guess": args.guess,
+            "brick": list(solver.brick), "bricks_per_wave": solver.bricks_per_wave,
+            "graph_steps": G, "electrodes": 64, "egm_precision": "bf16x3",
+            "sample_ring_rows": args.ring,
+        },
+        "speedup_vs_plain_fp64": value / plain_rate,
+        "plain_fp64": dict(voxel_steps_per_s=plain_rate, members=[v["member"] for v in val]),
+        "sustained": sustained,
+        "validate": val,
+        "egm_useful_tflop_total": egm_flop / 1e12,
+        "autotune": tune,
+        "shell": shell,
+        "latency_ms": ({"p50": 1e3 * serve_res["p50_s"], "p99": 1e3 * serve_res["p99_s"]}
+                       if serve_res else {"p50": None, "p99": None}),
+        "serve": serve_res,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
 
 
-def run_egm(args, torch, runtime):
-    """The electrogram of the 0.4 mm shell alone: its 64 electrodes x 2000 samples over the
-    199 446 voxels (BF16 hi/lo MFMA contraction with the precomputed lead field), synthetic
+def run_shell(args, torch, runtime, pre):
+    """The 0.4 mm shell alone (plain, chosen single, x16) with the full 40 000-step validation
+    of the chosen form against plain FP64 and the 64-electrode electrogram chain."""
+    from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
+    solver = runtime.SolverConfig(sweeps=max(args.sweeps, 0), guess=args.guess)
+    members = pre["members"]
+    chain = Chain(torch, runtime, members[:1], solver, ring=args.ring)
+    nspm = members[0]["prep"].steps_per_sample
+    chain.warm_egm(math.ceil(args.graph_steps / nspm) + 1)
+    t = timed_chain(torch, chain, args.steps, args.warmup, args.graph_steps)
+    N = members[0]["prep"].n_voxel
+    value = N * args.steps / t
+    chain.advance(members[0]["prep"].n_steps - chain.steps, args.graph_steps)
+    rec, rate, _ = validate_member(torch, runtime, chain, 0, LeadField)
+    cnt = chain.arena.counts()
+    prof = chain.arena.profile(50)
+    sweeps = chain.arena.sweeps()
+    chain.close()
+    legs = shell_legs(torch, runtime, solver, args.workers)
+    return {
+        "metric": METRIC, "value": value, "unit": "voxel-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"shell: MS ellipsoidal shell 25/20/18 mm, 3 mm wall, 4 vein "
+                               f"holes, 0.4 mm voxels (N={N}), dt 0.05 ms, CL 500 ms, 2000 ms, "
+                               f"64-electrode electrogram, alone", "n_voxel": N,
+                   "sweeps": sweeps, "guess": args.guess},
+        "speedup_vs_plain_fp64": value / rate, "validate": rec, "legs": legs,
+        "roofline": mid_sweep_roofline(cnt, prof, sweeps, 1e3 * t / args.steps, args.guess),
+        "cpu_baseline": None,
+    }
+
+
+def run_egm(args, torch, runtime, pre):
+    """The electrogram alone on the 0.4 mm shell: E electrodes x 2000 samples over the 199 446
+    voxels (BF16 hi/lo MFMA contraction with the precomputed lead field), synthetic
     plane-wave samples, --steps timed launches after --warmup; rel-L2 against the FP32 MFMA
     form on the same samples.  One launch reads the samples (4 T N bytes) and the BF16 hi/lo
-    lead field (4 E N) and writes the T x E result: at E = 64 the useful intensity (2 E T N
-    flop over those bytes) is ~32 flop/B, so HBM bounds it."""
+    lead field (4 E N) and writes the T x E result."""
+    from pyheartsim_amd import geometry as geo
     from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
-    g, work, preps = build_workload(range(1))
-    N = preps[0].n_voxel
-    el = work["electrodes"]
-    E, T = el.shape[0], 2000
-    lf = LeadField(el, g["voxel"], None, work["hm"]["c_voxel"], g["Delta"], g["neighbor_id_2d"])
-    x = torch.from_numpy(g["voxel"][:, 0].astype(np.float32)).cuda()
+    m = pre["members"][0]
+    N = m["prep"].n_voxel
+    E, T = args.electrodes, 2000
+    el = geo.fibonacci_electrodes(E, 15.0)
+    lf = LeadField(el, m["voxel"], None, m["c_voxel"], m["delta"], m["prep"].nb)
+    x = torch.from_numpy(m["voxel"][:, 0].astype(np.float32)).cuda()
     tt = torch.arange(T, device="cuda", dtype=torch.float32)[:, None]
     samples = torch.sigmoid(tt * 0.03 - (x[None, :] + 25.0) * 0.3)   # a plane wave in x
     e_ref = lf.run(samples, precision="fp32", normalize=False)
@@ -247,56 +585,55 @@ def run_egm(args, torch, runtime):
     st = torch.cuda.current_stream()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
     ev0.record(st)
     for _ in range(K):
         lf.run(samples, precision="bf16x3", normalize=False, out=out)
     ev1.record(st)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     ms = ev0.elapsed_time(ev1) / K
     rel = float(((out - e_ref).norm() / e_ref.norm()).item())
     useful = 2.0 * E * T * N
     nbytes = 4.0 * T * N + 4.0 * E * N + 4.0 * T * E
     gbs = nbytes / (ms * 1e-3) / 1e9
-    log(f"egm: E={E} T={T} N={N} {ms:.4f} ms/launch, rel_l2 {rel:.3e}, {gbs:.0f} GB/s")
+    issued = 3.0 * useful / (ms * 1e-3) / 1e12
+    mfma_bound = E >= 128
+    log(f"egm: E={E} T={T} N={N} {ms:.4f} ms/launch, rel_l2 {rel:.3e}, {gbs:.0f} GB/s, "
+        f"{issued:.0f} TF/s issued")
+    roof = ({"bound": "mfma", "achieved": issued, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+             "frac": issued / BF16_PEAK_TFLOPS} if mfma_bound else
+            {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": gbs / HBM_PEAK_GBS})
+    roof.update(kernel="egm_gemm_lds + egm_reduce (one electrogram)", bytes_per_launch=nbytes,
+                hbm_gbs=gbs, mfma_issued_tflops=issued,
+                traffic=_profile_record(PROFILE_TRAFFIC_EGM,
+                                        {"n_voxel": N, "electrodes": E, "samples": T},
+                                        "bytes_per_launch"))
     return {
         "metric": "electrogram useful TFLOP/s (2 E T N per launch, BF16 hi/lo MFMA)",
-        "value": useful / (ms * 1e-3) / 1e12,
-        "unit": "TFLOP/s",
+        "value": useful / (ms * 1e-3) / 1e12, "unit": "TFLOP/s",
         "n_gpus": 1, "steps": K, "warmup": max(args.warmup, 1), "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
         "data": "synthetic",
         "config": {"workload": f"egm: 0.4 mm shell lead field, E={E} electrodes, T={T} samples, "
                                f"N={N} voxels", "electrodes": E, "samples": T, "n_voxel": N},
-        "roofline": {"bound": "hbm", "kernel": "egm_gemm_lds + egm_reduce (one electrogram)",
-                     "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": nbytes,
-                     "traffic": _profile_record(PROFILE_TRAFFIC_EGM,
-                                                {"n_voxel": N, "electrodes": E, "samples": T},
-                                                "bytes_per_launch")},
-        "mfma_issued_tflops": 3.0 * useful / (ms * 1e-3) / 1e12,
-        "rel_l2_vs_fp32_mfma": rel,
-        "wall_s": wall,
+        "roofline": roof, "rel_l2_vs_fp32_mfma": rel, "cpu_baseline": None,
     }
 
 
-def run_precision(args, torch, runtime):
-    """The 0.2 mm shell with 3 scar blobs: FP32 sweeps with FP64-residual refinement, the
-    achieved CN residual, LAT against the plain FP64 form over a window, and the 256-electrode
-    x 2048-sample MFMA electrogram."""
+def run_precision(args, torch, runtime, pre):
+    """The 0.2 mm shell with 3 scar blobs (15 % of the wall): FP32 sweeps without and with
+    FP64-residual refinement, the achieved CN residual, LAT against the plain FP64 form over
+    the window, and the 256-electrode electrogram of the simulated samples (chosen chain)
+    against the plain chain."""
     from pyheartsim_amd import configs
     from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
-    t0 = time.perf_counter()
     w = configs.shell_fine(t_final=args.precision_ms)
     g = w["geometry"]
     P = configs.build_P_2d(g, w["sim"], w["hm"])
     prep = runtime.prepare(P.shape[0], P, g, w["sim"], w["arr"])
     N = prep.n_voxel
-    log(f"precision: N={N} steps={prep.n_steps} scar={int((prep.vflags & 1).sum())} setup "
-        f"{time.perf_counter()-t0:.1f}s")
     out = dict(workload="precision", n_voxel=N, steps=prep.n_steps)
-    plain = runtime.PlainSim(prep, tol=1e-15, samples=False)
+    plain = runtime.PlainSim(prep, tol=1e-15, samples=True, sample_h=False, sample_fp64=False)
     t0 = time.perf_counter()
     plain.run()
     torch.cuda.synchronize()
@@ -304,99 +641,128 @@ def run_precision(args, torch, runtime):
     out["plain"] = dict(voxel_steps_per_s=plain_rate, sweeps=plain.info()[0],
                         lambda_max=plain.info()[2])
     ref = plain.lat.cpu().numpy()
-    u_plain = plain.state()[0]          # FP64 state after the last step, caller order
+    u_plain = plain.state()[0]
+    el = w["electrodes"]
+    lf_p = LeadField(el, g["voxel"], None, w["hm"]["c_voxel"], g["Delta"], prep.nb)
+    e_plain = lf_p.run(plain.su, precision="fp32")
+    lf_p.close()
     del plain
-    # automatic sweep count (stability + accuracy rule) without refinement, and with three
-    # FP64-residual refinement passes of 12 sweeps (CN residual <= 1e-10)
+    torch.cuda.empty_cache()
+    value = ms_step = None
     for refine, rsw in ((0, 8), (3, 12)):
         cfg = runtime.SolverConfig(sweeps=max(args.sweeps, 0), guess=args.guess, refine=refine,
                                    refine_sweeps=rsw, diag=True)
-        arena = runtime.Arena(N, 1, cfg)
-        slot = arena.add(prep, samples=False)
-        lmax = arena.bounds(slot)[1]
+        members = [dict(prep=prep, voxel=g["voxel"], c_voxel=w["hm"]["c_voxel"],
+                        delta=g["Delta"], electrodes=el)]
+        chain = Chain(torch, runtime, members, cfg, ring=args.ring)
+        lmax = chain.arena.bounds(chain.slots[0])[1]
+        chain.warm_egm(math.ceil(args.graph_steps / prep.steps_per_sample) + 1)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        arena.step(prep.n_steps, graph=False)
+        chain.advance(prep.n_steps, args.graph_steps)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        res = arena.last_residual(slot)
-        _, _, lat = arena.outputs(slot)
-        u_end = arena.state(slot)[0]    # x after the simulation's last step
-        dvm = float((u_end.double() - u_plain).abs().max().item())
+        res = chain.arena.last_residual(chain.slots[0])
+        lat = chain.arena.lat(chain.slots[0]).cpu().numpy()
+        dvm = float((chain.arena.state(chain.slots[0])[0].double() - u_plain).abs().max().item())
+        e_c = chain.normalized_egm(0)
+        rel = float(((e_c - e_plain).norm() / e_plain.norm()).item())
         rate = N * prep.n_steps / dt
         out[f"refine{refine}"] = dict(
-            voxel_steps_per_s=rate, speedup_vs_plain=rate / plain_rate, sweeps=arena.sweeps(),
-            refine_sweeps=rsw if refine else 0,
-            stable_sweeps=runtime.stable_sweeps(lmax, args.guess, refine, rsw)[0],
+            voxel_steps_per_s=rate, speedup_vs_plain=rate / plain_rate,
+            sweeps=chain.arena.sweeps(), refine_sweeps=rsw if refine else 0,
             lambda_max=lmax, cn_rel_residual=res, max_abs_dVm_final=dvm,
-            lat_err_steps=lat_error(lat.cpu().numpy(), ref, prep.n_steps))
+            lat_err_steps=lat_error(lat, ref, prep.n_steps), egm_rel_l2_vs_plain_chain=rel)
         log(f"precision refine={refine}: {out[f'refine{refine}']}")
-        arena.close()
-    # electrogram: 256 electrodes x 2048 samples over all voxels, synthetic activation samples
-    E, T = 256, 2048
-    el = w["electrodes"]
-    lf = LeadField(el, g["voxel"], None, w["hm"]["c_voxel"], g["Delta"], g["neighbor_id_2d"])
-    x = torch.from_numpy(g["voxel"][:, 0].astype(np.float32)).cuda()
-    tt = torch.arange(T, device="cuda", dtype=torch.float32)[:, None]
-    samples = torch.sigmoid((tt * 0.03 - (x[None, :] + 25.0) * 0.3))   # a plane wave in x
-    e_ref = lf.run(samples, precision="fp32", normalize=False)
-    e_bf = lf.run(samples, precision="bf16x3", normalize=False)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(3):
-        lf.run(samples, precision="bf16x3", normalize=False, out=e_bf)
-    ev1.record()
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1) / 3
-    useful = 2.0 * E * T * N
-    out["egm"] = dict(electrodes=E, samples=T, ms=ms, useful_tflops=useful / ms / 1e9,
-                      mfma_issued_tflops=3 * useful / ms / 1e9,
-                      mfma_frac=3 * useful / ms / 1e9 / BF16_PEAK_TFLOPS,
-                      rel_l2_vs_fp32_mfma=float(((e_bf - e_ref).norm() / e_ref.norm()).item()))
-    log(f"precision egm: {out['egm']}")
-    return out
+        if refine:
+            value, ms_step = rate, 1e3 * dt / prep.n_steps
+        chain.close()
+        torch.cuda.empty_cache()
+    return {"metric": METRIC, "value": value, "unit": "voxel-steps/s", "n_gpus": 1,
+            "steps": prep.n_steps, "warmup": 0, "ms_per_step": ms_step,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": f"precision: 0.2 mm shell (N={N}) with 3 scar blobs, "
+                                   f"{args.precision_ms} ms, refine 3 x 12, 256 electrodes"},
+            "detail": out, "cpu_baseline": None, "roofline": None}
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--workload", choices=["shell_ms", "ragged", "precision", "egm"],
-                    default="shell_ms")
-    ap.add_argument("--ragged-sims", type=int, default=64)
-    ap.add_argument("--precision-ms", type=float, default=300.0)
+def run_serve(args, torch, runtime, pre):
+    solver = runtime.SolverConfig(sweeps=max(args.sweeps, 0), guess=args.guess)
+    res = serve_leg(torch, runtime, solver, pre["requests"], args.qps)
+    return {"metric": METRIC, "value": res["voxel_steps_per_s"], "unit": "voxel-steps/s",
+            "n_gpus": 1, "steps": None, "warmup": None, "ms_per_step": None,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": f"served: {len(pre['requests'])} requests at QPS {args.qps}"},
+            "latency_ms": {"p50": 1e3 * res["p50_s"], "p99": 1e3 * res["p99_s"]},
+            "serve": res, "cpu_baseline": None, "roofline": None}
+
+
+def parser():
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--workload", choices=["batched", "shell", "precision", "egm", "serve"],
+                    default="batched")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--sims", type=int, default=64, help="members of the batched workload")
     ap.add_argument("--sweeps", type=int, default=0,
-                    help="Chebyshev sweeps per CN solve; 0 = the arena's automatic count (the "
-                         "fewest that keep the step stable for the spectral bound, with margin), "
-                         "-1 = the fewest stable for the chosen warm start (shell workload at "
-                         "lambda_max 8.5: 6 with --guess 2, the automatic count; 5 with "
-                         "--guess 1, whose validation meets LAT <= 1 dt but not the 1e-3 "
-                         "electrogram bar)")
+                    help="Chebyshev sweeps per CN solve; 0 = the arena's automatic count")
     ap.add_argument("--guess", type=int, default=2,
                     help="warm start of the CN correction: 0 none, 1 previous step, 2 linear "
                          "extrapolation of the previous two")
-    ap.add_argument("--serve-requests", type=int, default=12)
-    ap.add_argument("--qps", type=float, default=0.5)
+    ap.add_argument("--graph-steps", type=int, default=100,
+                    help="steps per replayed hipGraph (and per electrogram contraction)")
+    ap.add_argument("--ring", type=int, default=16, help="sample ring rows per simulation")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--qps", type=float, default=4.0)
+    ap.add_argument("--electrodes", type=int, default=64, help="egm workload")
+    ap.add_argument("--precision-ms", type=float, default=300.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-from-step", type=int, default=3000)
+    ap.add_argument("--workers", type=int, default=0,
+                    help="host processes for building the workloads (0 = the host threads "
+                         "this job may use, at most 16)")
+    ap.add_argument("--only-headline", action="store_true",
+                    help="batched workload only: skip the shell legs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="run a device copy of this size first (HBM counter calibration)")
-    ap.add_argument("--graph-steps", type=int, default=50)
-    ap.add_argument("--no-validate", action="store_true")
-    ap.add_argument("--only-batched", action="store_true",
-                    help="run only the batched leg (short profiler runs)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    args = ap.parse_args()
+    return ap
+
+
+def run(argv=None) -> dict:
+    """Parse `argv` (default sys.argv), run the workload and return the result record
+    (main() prints it as one JSON line)."""
+    args = parser().parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the simulations "
               "shard by request across GPUs, see the multi-GPU design notes.", file=sys.stderr)
         sys.exit(2)
+    if args.steps <= 0 or args.warmup < 0:
+        print("--steps must be > 0 and --warmup >= 0", file=sys.stderr)
+        sys.exit(2)
+    if args.workers <= 0:
+        args.workers = min(16, host_threads())
+
+    # host-side workload construction first, in forked workers, before the GPU is touched
+    from pyheartsim_amd import configs
+    t0 = time.perf_counter()
+    pre = {}
+    if args.workload == "batched":
+        pre["kws"] = configs.ragged_params(args.sims)
+        pre["members"] = build_parallel(_batched_member, pre["kws"], args.workers)
+    elif args.workload in ("shell", "egm"):
+        pre["members"] = build_parallel(_shell_member, [0], 1)
+    if args.workload in ("batched", "serve") and args.serve_requests > 0:
+        from pyheartsim_amd import serve
+        pre["requests"] = serve.make_requests(args.serve_requests, args.qps, seed=11,
+                                              workers=args.workers)
+    log(f"host setup: {time.perf_counter()-t0:.1f}s ({args.workers} workers)")
 
     import torch
     from pyheartsim_amd import runtime
-    from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
-
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.calibrate_copy > 0:
         n = args.calibrate_copy * (1 << 20) // 4
@@ -407,186 +773,13 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
         del src, dst
-    if args.workload != "shell_ms":
-        fn = {"ragged": run_ragged, "precision": run_precision, "egm": run_egm}[args.workload]
-        print(json.dumps(fn(args, torch, runtime)), flush=True)
-        return
-    t_setup = time.perf_counter()
-    g, work, preps = build_workload(range(args.batch))
-    prep = preps[0]
-    N = prep.n_voxel
-    S_full = prep.n_steps
-    log(f"workload: N={N} steps={S_full} batch={args.batch} setup {time.perf_counter()-t_setup:.1f}s")
-    K, W = args.steps, args.warmup
-    G = args.graph_steps
-    sweeps = args.sweeps
-    if sweeps < 0:
-        sweeps, lam = fewest_stable_sweeps(runtime, preps, args.guess)
-        log(f"sweeps: {sweeps} (fewest stable for lambda_max = {lam:.4f}, guess {args.guess})")
-    solver = runtime.SolverConfig(sweeps=sweeps, guess=args.guess, brick=(4, 4, 4))
-    res = {}
+    fn = {"batched": run_batched, "shell": run_shell, "precision": run_precision,
+          "egm": run_egm, "serve": run_serve}[args.workload]
+    return fn(args, torch, runtime, pre)
 
-    ks = (K // G) * G
-    u_mid = h_mid = step_mid = None
-    if not args.only_batched:
-        # ---- plain FP64, one simulation
-        plain = runtime.PlainSim(prep, tol=1e-15, samples=not args.no_validate, sample_h=False,
-                                 sample_fp64=False)
-        sweeps_plain = plain.info()[0]
-        t = timed(torch, plain.run, K, W, 200)
-        res["plain"] = dict(voxel_steps_per_s=N * K / t, ms_per_step=1e3 * t / K, sweeps=sweeps_plain)
-        log(f"plain fp64: {res['plain']}")
-        u_mid, h_mid = plain.state()
-        u_mid = u_mid.cpu().numpy()
-        h_mid = h_mid.cpu().numpy()
-        step_mid = W + K
 
-        # ---- chosen form, one simulation
-        single = runtime.Arena(N, 1, solver)
-        slot = single.add(prep, samples=not args.no_validate)
-        t = timed(torch, lambda n: single.step(n, graph=True), (K // G) * G, (W // G) * G, G)
-        res["single"] = dict(voxel_steps_per_s=N * ks / t, ms_per_step=1e3 * t / ks)
-        log(f"chosen single: {res['single']}")
-
-        # ---- validation: run both to the end of the 2000 ms simulation
-        if not args.no_validate:
-            t0 = time.perf_counter()
-            plain.run()
-            while single.steps_done(slot) < S_full:
-                single.step(G, graph=True)
-            torch.cuda.synchronize()
-            log(f"validation runs finished in {time.perf_counter()-t0:.1f}s")
-            su_c, _, lat_c = single.outputs(slot)
-            lat_p = plain.lat.cpu().numpy()
-            lat_cn = lat_c.cpu().numpy()
-            le = lat_error(lat_cn, lat_p, S_full)
-            T = prep.n_saved()
-            vm = float((su_c[:T] - plain.su[:T]).abs().max().item())
-            res["validate"] = dict(lat_err_steps=le, max_abs_dVm=vm, beats=int((lat_p[:, 0] >= 0).sum()))
-            log(f"validation: {res['validate']}")
-            # electrogram: chosen (BF16 hi/lo on compact-order samples) vs FP32 MFMA on plain samples
-            el = work["electrodes"]
-            c = work["hm"]["c_voxel"]
-            lf_c = LeadField(el, g["voxel"], None, c, g["Delta"], g["neighbor_id_2d"],
-                             perm=single.slots[slot].perm)
-            lf_p = LeadField(el, g["voxel"], None, c, g["Delta"], g["neighbor_id_2d"])
-            samples_c = single.slots[slot].su[:T]
-            e_ref = lf_p.run(plain.su[:T], precision="fp32")
-            e_c = lf_c.run(samples_c, precision="bf16x3")
-            torch.cuda.synchronize()
-            rel = float(((e_c - e_ref).norm() / e_ref.norm()).item())
-            st = torch.cuda.current_stream()
-            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            ev0.record(st)
-            reps = 5
-            for _ in range(reps):
-                lf_c.run(samples_c, precision="bf16x3", normalize=False)
-            ev1.record(st)
-            torch.cuda.synchronize()
-            ms = ev0.elapsed_time(ev1) / reps
-            E = el.shape[0]
-            useful = 2.0 * E * T * N
-            egm_bytes = 4.0 * T * N + 4.0 * E * N + 4.0 * T * E
-            res["egm"] = dict(rel_l2=rel, ms=ms, electrodes=E, samples=T,
-                              useful_tflops=useful / ms / 1e9,
-                              mfma_issued_tflops=3 * useful / ms / 1e9,
-                              mfma_issued_frac=3 * useful / ms / 1e9 / BF16_PEAK_TFLOPS,
-                              hbm_gbs=egm_bytes / ms / 1e6,
-                              hbm_frac=egm_bytes / ms / 1e6 / HBM_PEAK_GBS,
-                              reference_equivalent_tflops=2.0 * E * T * 3 * N / ms / 1e9)
-            log(f"egm: {res['egm']}")
-            del plain, single
-            torch.cuda.empty_cache()
-        else:
-            del plain, single
-
-    # ---- chosen form, batched
-    B = len(preps)
-    arena = runtime.Arena(N * B, B, solver)
-    for p in preps:
-        arena.add(p, samples=False)
-    cnt = arena.counts()
-    sweeps_used = arena.sweeps()
-    t = timed(torch, lambda n: arena.step(n, graph=True), ks, (W // G) * G, G)
-    value = cnt["voxels"] * ks / t
-    ms_step = 1e3 * t / ks
-    prof = arena.profile(50)
-    vf = cnt["vf_voxels"]
-    # algorithmic bytes of one Chebyshev sweep launch: read d_k, d_{k-1}, r (FP32) and write
-    # d_{k+1} per voxel, the flag byte of scar/temporary-block simulations, and the 192-byte
-    # brick records (neighbour values gathered across brick faces are re-reads, not counted)
-    bytes_sweep = cnt["voxels"] * 16 + vf + cnt["bricks"] * BRICK_RECORD_BYTES
-    achieved = bytes_sweep / (prof["sweep"] * 1e-3) / 1e9
-    log(f"batched: {value:.4g} voxel-steps/s, {ms_step:.4f} ms/step, kernels {prof}, counts {cnt}")
-
-    del arena
-    torch.cuda.empty_cache()
-
-    # ---- served form: open-loop Poisson stream, continuous batching
-    serve_res = None
-    if args.serve_requests > 0 and not args.only_batched:
-        from pyheartsim_amd import serve
-        t0 = time.perf_counter()
-        reqs = serve.make_requests(args.serve_requests, args.qps, seed=11)
-        log(f"serve: {len(reqs)} requests generated in {time.perf_counter()-t0:.1f}s, "
-            f"N {min(r.prep.n_voxel for r in reqs)}-{max(r.prep.n_voxel for r in reqs)}")
-        srv = serve.Server(max_voxels=min(60_000_000, sum(r.prep.n_voxel for r in reqs)),
-                           max_sims=64, hbm_budget_bytes=200e9, solver=solver, ring=256,
-                           chunk_steps=100)
-        st = srv.run(reqs, progress=lambda r: log(f"  served rid={r['rid']} N={r['n_voxel']} "
-                                                   f"steps={r['steps']} latency={r['latency']:.2f}s"))
-        serve_res = dict(qps=args.qps, requests=st.completed, p50_s=st.p50, p99_s=st.p99,
-                         voxel_steps_per_s=st.throughput, wall_s=st.wall,
-                         workload="Poisson arrivals; shells of 2e4-2e6 voxels (log-uniform) at "
-                                  "0.4 mm, MS/AP 50/50, 500-5000 ms, 32-256 electrodes")
-        log(f"serve: {serve_res}")
-
-    cpu = None
-    if args.cpu_seconds > 0 and u_mid is not None:
-        cpu = cpu_baseline(work, u_mid, h_mid, step_mid, args.cpu_seconds)
-        log(f"cpu: {cpu}")
-
-    line = {
-        "metric": "voxel-steps/s at LAT err<=1dt (chosen vs plain FP64); p50/p99 sim latency",
-        "value": value,
-        "unit": "voxel-steps/s",
-        "n_gpus": 1,
-        "steps": ks,
-        "warmup": (W // G) * G,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"shell_ms: MS ellipsoidal LA shell 25/20/18 mm, 3 mm wall, 4 vein holes, "
-                        f"0.4 mm voxels (N={N}), dt 0.05 ms, CL 500 ms, x{B} simulations packed "
-                        f"(different seeded pacing sites)",
-            "n_voxel": N, "batch": B, "steps_per_sim": S_full, "sweeps": sweeps_used,
-            "sweep_rule": ("fewest stable for the spectral bound" if args.sweeps < 0 else
-                           "automatic (stable with margin)" if args.sweeps == 0 else "fixed"),
-            "guess": args.guess, "brick": [4, 4, 4], "graph_steps": G,
-        },
-        "speedup_vs_plain_fp64": (value / res["plain"]["voxel_steps_per_s"]
-                                  if "plain" in res else None),
-        "plain_fp64": res.get("plain"),
-        "single": res.get("single"),
-        "validate": res.get("validate"),
-        "egm": res.get("egm"),
-        "latency_ms": ({"p50": 1e3 * serve_res["p50_s"], "p99": 1e3 * serve_res["p99_s"]}
-                       if serve_res else {"p50": None, "p99": None}),
-        "serve": serve_res,
-        "roofline": {
-            "bound": "hbm", "kernel": "arena_sweep (Chebyshev sweep, FP32)",
-            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(cnt),
-            "bytes_per_launch": bytes_sweep, "avg_launch_ms": prof["sweep"],
-            "kernels_ms": prof,
-        },
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
+def main():
+    print(json.dumps(run()), flush=True)
 
 
 if __name__ == "__main__":

@@ -186,6 +186,10 @@ int phs_arena_step(phs_arena* a, int32_t n_steps, void* stream);
 /* Record n_steps steps into a HIP graph once and replay it on later calls with the
  * same n_steps (the launch pattern does not depend on which slots are resident). */
 int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream);
+/* Capture (without launching) the graphs that phs_arena_step_graph replays for n_steps
+ * steps, for both launch parities, so that a later replay of that length does no capture.
+ * Host only; the graphs stay valid until the set of resident simulations changes. */
+int phs_arena_graph_prepare(phs_arena* a, int32_t n_steps);
 /* Host-side view of a slot: steps done, voxels, bricks, first compact index. */
 int phs_arena_slot_info(phs_arena* a, int32_t slot, int64_t* steps_done, int32_t* n_voxel,
                         int32_t* n_bricks, int64_t* voxel_offset);

@@ -8,7 +8,8 @@
  *   operator   :21-111   CSR L with L[n,n] = c_n/(4 Delta^2) * -(sum_k P[n,k]),
  *                        L[n,m_k] = c_n/(4 Delta^2) * P[n,k] for present face neighbours
  *                        (P = 4 delta_k for D = I; the anisotropy terms are zero)
- *   CN step    :113-142  A = I - dt/2 L, b = u* + dt/2 L u*, BiCGSTAB from x0 = u*
+ *   CN step    :113-142  A = I - dt/2 L formed as its own CSR matrix every step (:131-133),
+ *                        b = u* + dt/2 L u* (:134), BiCGSTAB on A from x0 = u* (:137)
  *                        stopping at ||r|| < max(rtol ||b||, atol) (scipy's rule; the
  *                        reference passes atol = 1e-6 and keeps rtol = 1e-5), maxiter 1000
  * with the pacing schedule, scar / temporary block and sampling of
@@ -84,7 +85,7 @@ static void free_L(csr* L) {
   free(L->val);
 }
 
-/* y = x - s * (L x)  (s = dt/2 gives A x; s = -dt/2 gives b from u*) */
+/* y = x - s * (L x)  (s = -dt/2 gives b = u* + dt/2 L u*) */
 static void apply(const csr* L, double s, const double* x, double* y) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < L->n; ++i) {
@@ -94,6 +95,36 @@ static void apply(const csr* L, double s, const double* x, double* y) {
   }
 }
 
+/* y = M x */
+static void spmv(const csr* M, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < M->n; ++i) {
+    double acc = 0.0;
+    for (int k = M->rowptr[i]; k < M->rowptr[i + 1]; ++k) acc += M->val[k] * x[M->col[k]];
+    y[i] = acc;
+  }
+}
+
+/* A = I - hdt L as a new CSR matrix with L's pattern (build_L stores the diagonal first in
+ * every row), in freshly allocated arrays, as scipy forms I - dt/2 L every step. */
+static void form_A(const csr* L, double hdt, csr* A) {
+  const int n = L->n;
+  const int32_t nnz = L->rowptr[n];
+  A->n = n;
+  A->rowptr = (int32_t*)malloc(sizeof(int32_t) * (n + 1));
+  A->col = (int32_t*)malloc(sizeof(int32_t) * (size_t)nnz);
+  A->val = (double*)malloc(sizeof(double) * (size_t)nnz);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    A->rowptr[i] = L->rowptr[i];
+    for (int k = L->rowptr[i]; k < L->rowptr[i + 1]; ++k) {
+      A->col[k] = L->col[k];
+      A->val[k] = (L->col[k] == i ? 1.0 : 0.0) - hdt * L->val[k];
+    }
+  }
+  A->rowptr[n] = nnz;
+}
+
 static double dot(const double* a, const double* b, int n) {
   double s = 0.0;
 #pragma omp parallel for reduction(+ : s) schedule(static)
@@ -101,15 +132,15 @@ static double dot(const double* a, const double* b, int n) {
   return s;
 }
 
-/* Unpreconditioned BiCGSTAB (the algorithm of scipy.sparse.linalg.bicgstab with M = I). */
-static int bicgstab(const csr* L, double hdt, const double* b, double* x, double rtol, double atol,
+/* Unpreconditioned BiCGSTAB on A (the algorithm of scipy.sparse.linalg.bicgstab with M = I). */
+static int bicgstab(const csr* A, const double* b, double* x, double rtol, double atol,
                     int maxiter, double* work) {
-  const int n = L->n;
+  const int n = A->n;
   double *r = work, *rt = work + n, *p = work + 2 * (size_t)n, *v = work + 3 * (size_t)n,
          *s = work + 4 * (size_t)n, *t = work + 5 * (size_t)n;
   const double bn = sqrt(dot(b, b, n));
   const double tol = fmax(atol, rtol * bn);
-  apply(L, hdt, x, v);
+  spmv(A, x, v);
 #pragma omp parallel for
   for (int i = 0; i < n; ++i) {
     r[i] = b[i] - v[i];
@@ -127,7 +158,7 @@ static int bicgstab(const csr* L, double hdt, const double* b, double* x, double
 #pragma omp parallel for
       for (int i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
     }
-    apply(L, hdt, p, v);
+    spmv(A, p, v);
     alpha = rho / dot(rt, v, n);
 #pragma omp parallel for
     for (int i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
@@ -136,7 +167,7 @@ static int bicgstab(const csr* L, double hdt, const double* b, double* x, double
       for (int i = 0; i < n; ++i) x[i] += alpha * p[i];
       return 0;
     }
-    apply(L, hdt, s, t);
+    spmv(A, s, t);
     omega = dot(t, s, n) / dot(t, t, n);
 #pragma omp parallel for
     for (int i = 0; i < n; ++i) {
@@ -150,14 +181,19 @@ static int bicgstab(const csr* L, double hdt, const double* b, double* x, double
 }
 
 /* Advance (u, h) by n_steps steps; sched[i] bits: 1 S1 on, 2 S2 on, 4 temporary block.
- * Returns the number of CN solves that hit maxiter. */
+ * rebuild != 0 forms A = I - dt/2 L anew every step (the reference's cost); otherwise once
+ * per call for each of the two operators.  Returns the number of CN solves that hit maxiter. */
 int cpu_simulate(const cpu_desc* d, const uint8_t* sched, int64_t n_steps, double* u, double* h,
-                 int threads) {
+                 int threads, int rebuild) {
   if (threads > 0) omp_set_num_threads(threads);
   const int n = d->n;
-  csr L0, L1;
+  csr L0, L1, A0, A1;
   build_L(d, 0, &L0);
   build_L(d, 1, &L1);
+  if (!rebuild) {
+    form_A(&L0, d->dt / 2.0, &A0);
+    form_A(&L1, d->dt / 2.0, &A1);
+  }
   double* us = (double*)malloc(sizeof(double) * n);
   double* b = (double*)malloc(sizeof(double) * n);
   double* work = (double*)malloc(sizeof(double) * 6 * (size_t)n);
@@ -177,14 +213,25 @@ int cpu_simulate(const cpu_desc* d, const uint8_t* sched, int64_t n_steps, doubl
       h[i] = h[i] + (dt / 6.0) * (k1h + 2 * k2h + 2 * k3h + k4h);
     }
     const csr* L = (sc & 4) ? &L1 : &L0;
+    csr Astep;
+    const csr* A = (sc & 4) ? &A1 : &A0;
+    if (rebuild) {
+      form_A(L, dt / 2.0, &Astep);
+      A = &Astep;
+    }
     apply(L, -dt / 2.0, us, b);
     memcpy(u, us, sizeof(double) * n);
-    if (bicgstab(L, dt / 2.0, b, u, d->rtol, d->atol, d->maxiter, work)) ++fails;
+    if (bicgstab(A, b, u, d->rtol, d->atol, d->maxiter, work)) ++fails;
+    if (rebuild) free_L(&Astep);
   }
   free(us);
   free(b);
   free(work);
   free_L(&L0);
   free_L(&L1);
+  if (!rebuild) {
+    free_L(&A0);
+    free_L(&A1);
+  }
   return fails;
 }

@@ -44,7 +44,7 @@ def load():
         _lib = C.CDLL(str(LIB))
         _lib.cpu_simulate.restype = C.c_int
         _lib.cpu_simulate.argtypes = [C.POINTER(_Desc), C.c_void_p, C.c_int64, C.c_void_p,
-                                      C.c_void_p, C.c_int]
+                                      C.c_void_p, C.c_int, C.c_int]
     return _lib
 
 
@@ -93,10 +93,14 @@ class CpuSim:
             out[i] = (1 if on1 else 0) | (2 if on2 else 0) | (4 if 0 <= t < 350 * dt else 0)
         return out
 
-    def run(self, n_steps, threads=0):
+    def run(self, n_steps, threads=0, rebuild=True):
+        """Advance n_steps steps on `threads` OpenMP threads (0 = the runtime's default).
+        rebuild: form A = I - dt/2 L anew every step, as legacy/compute_simulation_cpu.py:131-133
+        does; False forms it once per call.  Returns the count of solves that hit maxiter."""
         sched = self.schedule(n_steps)
         fails = load().cpu_simulate(C.byref(self.desc), sched.ctypes.data, n_steps,
-                                    self.u.ctypes.data, self.h.ctypes.data, threads)
+                                    self.u.ctypes.data, self.h.ctypes.data, threads,
+                                    1 if rebuild else 0)
         self.step += n_steps
         return fails
 

@@ -96,6 +96,7 @@ SIGNATURES = {
     "phs_arena_remove": (C.c_int, [_P, _I32, _P]),
     "phs_arena_step": (C.c_int, [_P, _I32, _P]),
     "phs_arena_step_graph": (C.c_int, [_P, _I32, _P]),
+    "phs_arena_graph_prepare": (C.c_int, [_P, _I32]),
     "phs_arena_slot_info": (C.c_int, [_P, _I32, _PI64, _PI32, _PI32, _PI64]),
     "phs_arena_perm": (C.c_int, [_P, _I32, _P]),
     "phs_arena_state": (C.c_int, [_P, _I32, _P, _P, _P]),

@@ -161,16 +161,26 @@ def shell_fine(delta: float = 0.2, t_final: float = 2000.0, seed: int = 0,
     return w
 
 
-def ragged(n_sims: int = 64, t_final: float = 1000.0, seed: int = 7, delta: float = 0.4,
-           axis_range=(15.0, 30.0), wall_range=(2.0, 4.0)):
+def ragged_params(n_sims: int = 64, t_final: float = 1000.0, seed: int = 7, delta: float = 0.4,
+                  axis_range=(15.0, 30.0), wall_range=(2.0, 4.0)):
+    """Keyword arguments of `shell_ms` for each member of the batched workload: semi-axes
+    U(15, 30) mm (sorted, largest first), wall U(2, 4) mm, cycle length U(250, 600) ms, 0-3
+    scar blobs and a seeded pacing site, all drawn from one generator in member order."""
     rng = np.random.default_rng(seed)
     out = []
-    for i in range(n_sims):
+    for _ in range(n_sims):
         ax = np.sort(rng.uniform(*axis_range, size=3))[::-1]
         wall = rng.uniform(*wall_range)
         cl = rng.uniform(250.0, 600.0)
         n_scar = int(rng.integers(0, 4))
-        w = shell_ms(delta=delta, t_final=t_final, seed=int(rng.integers(1 << 30)),
-                     semi_axes=tuple(ax), wall=wall, cycle_length=cl, n_scar=n_scar)
-        out.append(w)
+        out.append(dict(delta=delta, t_final=t_final, seed=int(rng.integers(1 << 30)),
+                        semi_axes=tuple(float(a) for a in ax), wall=float(wall),
+                        cycle_length=float(cl), n_scar=n_scar))
     return out
+
+
+def ragged(n_sims: int = 64, t_final: float = 1000.0, seed: int = 7, delta: float = 0.4,
+           axis_range=(15.0, 30.0), wall_range=(2.0, 4.0)):
+    """The 64 uneven shells of the batched workload (see `ragged_params`)."""
+    return [shell_ms(**kw) for kw in ragged_params(n_sims, t_final, seed, delta, axis_range,
+                                                   wall_range)]

@@ -737,35 +737,61 @@ int phs_arena_step(phs_arena* a, int32_t n_steps, void* stream) {
   return PHS_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The cached graph of n_steps steps starting at launch parity par0, captured (not launched)
+// when the cache holds none for the arena's current launch pattern.
+int ensure_graph(phs_arena* a, int32_t n_steps, int32_t par0) {
+  const int64_t grid = grid_for(a);
+  phs_arena::Graph& G = a->g[par0];
+  if (G.exec && G.steps == n_steps && G.grid == grid && G.high == a->brick_high &&
+      G.sweeps == a->sweeps_eff && G.flav == a->flav)
+    return PHS_OK;
+  if (G.exec) hipGraphExecDestroy(G.exec);
+  if (G.graph) hipGraphDestroy(G.graph);
+  G.exec = nullptr;
+  G.graph = nullptr;
+  if (!a->cap_stream) PHS_HIP_CHECK(hipStreamCreateWithFlags(&a->cap_stream, hipStreamNonBlocking));
+  PHS_HIP_CHECK(hipStreamBeginCapture(a->cap_stream, hipStreamCaptureModeThreadLocal));
+  const int32_t par_saved = a->par;
+  a->par = par0;
+  for (int i = 0; i < n_steps; ++i) launch_step(a, a->cap_stream);  // flips a->par
+  a->par = par_saved;  // nothing ran: only a replay advances the parity
+  hipGraph_t g = nullptr;
+  PHS_HIP_CHECK(hipStreamEndCapture(a->cap_stream, &g));
+  G.graph = g;
+  PHS_HIP_CHECK(hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0));
+  G.steps = n_steps;
+  G.grid = grid;
+  G.high = a->brick_high;
+  G.sweeps = a->sweeps_eff;
+  G.flav = a->flav;
+  return PHS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phs_arena_graph_prepare(phs_arena* a, int32_t n_steps) {
+  if (!a) return phs::fail(PHS_ERR_ARG, "handle is NULL");
+  if (n_steps <= 0) return phs::fail(PHS_ERR_ARG, "n_steps must be > 0");
+  if (a->brick_high == 0) return PHS_OK;
+  for (int32_t p = 0; p < 2; ++p)
+    if (int rc = ensure_graph(a, n_steps, p)) return rc;
+  return PHS_OK;
+}
+
 int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream) {
   if (!a) return phs::fail(PHS_ERR_ARG, "handle is NULL");
   if (n_steps <= 0) return phs::fail(PHS_ERR_ARG, "n_steps must be > 0");
   if (a->brick_high == 0) return PHS_OK;
   hipStream_t s = (hipStream_t)stream;
-  const int64_t grid = grid_for(a);
   const int32_t par0 = a->par;
-  phs_arena::Graph& G = a->g[par0];
-  if (!G.exec || G.steps != n_steps || G.grid != grid || G.high != a->brick_high ||
-      G.sweeps != a->sweeps_eff || G.flav != a->flav) {
-    if (G.exec) hipGraphExecDestroy(G.exec);
-    if (G.graph) hipGraphDestroy(G.graph);
-    G.exec = nullptr;
-    G.graph = nullptr;
-    if (!a->cap_stream) PHS_HIP_CHECK(hipStreamCreateWithFlags(&a->cap_stream, hipStreamNonBlocking));
-    PHS_HIP_CHECK(hipStreamBeginCapture(a->cap_stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < n_steps; ++i) launch_step(a, a->cap_stream);  // flips a->par
-    a->par = par0;  // nothing ran yet: the replay below advances it
-    hipGraph_t g = nullptr;
-    PHS_HIP_CHECK(hipStreamEndCapture(a->cap_stream, &g));
-    G.graph = g;
-    PHS_HIP_CHECK(hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0));
-    G.steps = n_steps;
-    G.grid = grid;
-    G.high = a->brick_high;
-    G.sweeps = a->sweeps_eff;
-    G.flav = a->flav;
-  }
-  PHS_HIP_CHECK(hipGraphLaunch(G.exec, s));
+  if (int rc = ensure_graph(a, n_steps, par0)) return rc;
+  PHS_HIP_CHECK(hipGraphLaunch(a->g[par0].exec, s));
   a->par ^= n_steps & 1;
   for (auto& h : a->sh)
     if (h.used) h.steps_done = std::min<int64_t>(h.n_steps, h.steps_done + n_steps);

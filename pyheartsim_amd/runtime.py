@@ -332,6 +332,11 @@ class Arena:
         fn = self.lib.phs_arena_step_graph if graph else self.lib.phs_arena_step
         _lib.check(fn(self.handle, n, _lib.stream_ptr(stream)), "phs_arena_step")
 
+    def prepare_graph(self, n: int):
+        """Capture the n-step graphs of both launch parities now, so that replays of n steps
+        do no capture (phs_arena_graph_prepare)."""
+        _lib.check(self.lib.phs_arena_graph_prepare(self.handle, n), "phs_arena_graph_prepare")
+
     def steps_done(self, slot: int) -> int:
         s = C.c_int64()
         _lib.check(self.lib.phs_arena_slot_info(self.handle, slot, C.byref(s), None, None, None))
@@ -391,6 +396,12 @@ class Arena:
         _lib.check(fn(_lib.ptr(dst), _lib.ptr(src), _lib.ptr(perm_dev), rows, n, _lib.stream_ptr()),
                    "scatter")
         return dst
+
+    def lat(self, slot: int):
+        """Activation times (max_beats x N int32, -1 = none) in caller voxel order."""
+        o = self.slots[slot]
+        perm = self.torch.from_numpy(o.perm).to("cuda")
+        return self._scatter(o.lat, perm, self.lib.phs_scatter_rows_i32)
 
     def outputs(self, slot: int):
         """(samples_u, samples_h, lat) in caller voxel order (device tensors)."""

@@ -70,12 +70,12 @@ def shell_for_voxels(n_target: float, seed: int, delta: float = 0.4):
     return geo.shell(axes, wall, delta, 4, hole, seed)
 
 
-def make_requests(n_requests: int, qps: float, seed: int = 11, n_range=(2e4, 2e6),
-                  duration_range=(500.0, 5000.0), electrode_range=(32, 256), delta: float = 0.4,
-                  dt_ms: float = 0.05):
-    """The served workload: Poisson arrivals at `qps`; each request a seeded shell of
-    log-uniform voxel count, Mitchell-Schaeffer or Aliev-Panfilov (50/50), uniform
-    duration and electrode count."""
+def request_params(n_requests: int, qps: float, seed: int = 11, n_range=(2e4, 2e6),
+                   duration_range=(500.0, 5000.0), electrode_range=(32, 256), delta: float = 0.4,
+                   dt_ms: float = 0.05):
+    """The draws of the served workload, one dict per request, from one generator in request
+    order: Poisson arrival times at `qps`, a log-uniform voxel count, a shell seed,
+    Mitchell-Schaeffer or Aliev-Panfilov (50/50), a uniform duration and electrode count."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -83,20 +83,52 @@ def make_requests(n_requests: int, qps: float, seed: int = 11, n_range=(2e4, 2e6
         t += rng.exponential(1.0 / qps)
         n_target = math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))
         s = int(rng.integers(1 << 30))
-        g = shell_for_voxels(n_target, s, delta)
         model = int(rng.integers(2))
         dur = float(rng.uniform(*duration_range))
         n_el = int(rng.integers(electrode_range[0], electrode_range[1] + 1))
-        ts = 6 if model == 1 else 1
-        w = configs.shell_ms(delta=delta, t_final=dur, seed=s, n_electrodes=n_el, dt=dt_ms,
-                             heart_model_flag=model, geometry=g,
-                             semi_axes=g["semi_axes"], wall=g["wall"])
-        P = configs.build_P_2d(g, w["sim"], w["hm"])
-        prep = runtime.prepare(P.shape[0], P, g, w["sim"], w["arr"])
-        el = geo.fibonacci_electrodes(n_el, 0.6 * min(g["semi_axes"]))
-        out.append(Request(rid, t, prep, g["voxel"], g["neighbor_id_2d"], g["Delta"],
-                           w["hm"]["c_voxel"] / ts, el, model, dur))
+        out.append(dict(rid=rid, arrival=t, n_target=n_target, seed=s, model=model,
+                        duration_ms=dur, n_electrodes=n_el, delta=delta, dt_ms=dt_ms))
     return out
+
+
+def build_request(p: dict) -> Request:
+    """The Request of one `request_params` entry: its seeded shell, parameters, schedule and
+    electrode set (host work only)."""
+    g = shell_for_voxels(p["n_target"], p["seed"], p["delta"])
+    model = p["model"]
+    ts = 6 if model == 1 else 1
+    w = configs.shell_ms(delta=p["delta"], t_final=p["duration_ms"], seed=p["seed"],
+                         n_electrodes=p["n_electrodes"], dt=p["dt_ms"], heart_model_flag=model,
+                         geometry=g, semi_axes=g["semi_axes"], wall=g["wall"])
+    P = configs.build_P_2d(g, w["sim"], w["hm"])
+    prep = runtime.prepare(P.shape[0], P, g, w["sim"], w["arr"])
+    el = geo.fibonacci_electrodes(p["n_electrodes"], 0.6 * min(g["semi_axes"]))
+    return Request(p["rid"], p["arrival"], prep, g["voxel"], g["neighbor_id_2d"][:, :6].copy(),
+                   g["Delta"], w["hm"]["c_voxel"] / ts, el, model, p["duration_ms"])
+
+
+def make_requests(n_requests: int, qps: float, seed: int = 11, n_range=(2e4, 2e6),
+                  duration_range=(500.0, 5000.0), electrode_range=(32, 256), delta: float = 0.4,
+                  dt_ms: float = 0.05, workers: int = 1):
+    """The served workload: Poisson arrivals at `qps`; each request a seeded shell of
+    log-uniform voxel count, Mitchell-Schaeffer or Aliev-Panfilov (50/50), uniform
+    duration and electrode count.  workers > 1 builds the requests in forked processes (call
+    it before the process uses the GPU)."""
+    params = request_params(n_requests, qps, seed, n_range, duration_range, electrode_range,
+                            delta, dt_ms)
+    return parallel_map(build_request, params, workers)
+
+
+def parallel_map(fn, items, workers: int):
+    """[fn(x) for x in items], in forked worker processes when workers > 1 (host-only work;
+    the parent must not have initialised the GPU)."""
+    if workers <= 1 or len(items) <= 1:
+        return [fn(x) for x in items]
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    with cf.ProcessPoolExecutor(max_workers=min(workers, len(items)),
+                                mp_context=mp.get_context("fork")) as ex:
+        return list(ex.map(fn, items))
 
 
 @dataclass
@@ -138,6 +170,12 @@ class Server:
         self.keep = keep_results
 
     def _admit(self, r: Request, now: float) -> _Active | None:
+        # A chunk of steps must produce at most `ring` samples: the ring holds two halves,
+        # one being filled while the other waits for its contraction.
+        per_chunk = math.ceil(self.chunk / max(r.prep.steps_per_sample, 1))
+        if per_chunk > self.ring:
+            raise ValueError(f"request {r.rid}: {per_chunk} samples per chunk of {self.chunk} "
+                             f"steps exceed the half ring of {self.ring} rows")
         fp = r.footprint_bytes(2 * self.ring)
         if self.in_use + fp > self.budget:
             return None
