@@ -199,17 +199,17 @@ class Chain:
     each chunk, contracts the sample rows the chunk produced (read from each simulation's
     sample ring) into that simulation's electrogram on the same stream."""
 
-    def __init__(self, torch, runtime, members, solver, ring=32, egm=True, precision="bf16x3"):
-        from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
+    def __init__(self, torch, runtime, members, solver, ring=32, egm=True):
+        from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField, StreamContraction
         self.torch = torch
         self.members = members
         self.ring = ring
-        self.precision = precision
         n_vox = sum(m["prep"].n_voxel for m in members)
         self.arena = runtime.Arena(n_vox, len(members), solver)
         self.slots = [self.arena.add(m["prep"], samples=egm, ring=ring if egm else 0)
                       for m in members]
         self.lfs, self.egms, self.rows = [], [], []
+        self.stream = StreamContraction()
         if egm:
             for m, s in zip(members, self.slots):
                 nb6 = m["prep"].nb
@@ -227,16 +227,26 @@ class Chain:
         nsp = p.steps_per_sample
         return min(p.n_saved(), (done + nsp - 1) // nsp if nsp > 0 else done)
 
-    def contract(self):
+    def _jobs(self, upto_of):
+        jobs = []
         for i, lf in enumerate(self.lfs):
-            upto = self.produced(i)
+            upto = upto_of(i)
             su = self.arena.slots[self.slots[i]].su
-            while self.rows[i] < upto:
-                r0 = self.rows[i] % self.ring
-                n = min(upto - self.rows[i], self.ring - r0)
-                lf.run(su[r0:r0 + n], precision=self.precision, normalize=False,
-                       out=self.egms[i][self.rows[i]:self.rows[i] + n])
-                self.rows[i] += n
+            r = self.rows[i]
+            while r < upto:
+                r0 = r % self.ring
+                n = min(upto - r, self.ring - r0, 16)
+                jobs.append((lf, su[r0:r0 + n], self.egms[i][r:r + n]))
+                r += n
+        return jobs
+
+    def contract(self):
+        """Electrogram rows of every sample produced since the last call: one streamed
+        contraction over all simulations (phs_egm_stream)."""
+        jobs = self._jobs(self.produced)
+        self.stream.run(jobs)
+        for i in range(len(self.lfs)):
+            self.rows[i] = max(self.rows[i], self.produced(i))
 
     def advance(self, n, chunk):
         """n steps in graph chunks of `chunk` (the remainder as eager steps), contracting
@@ -251,13 +261,13 @@ class Chain:
                 self.contract()
 
     def warm_egm(self, rows):
-        """One contraction of `rows` rows per member into scratch, so the split-K workspaces
-        are sized before any timed region."""
-        for i, lf in enumerate(self.lfs):
-            su = self.arena.slots[self.slots[i]].su
-            scratch = self.torch.empty((rows, self.egms[i].shape[1]), dtype=self.torch.float32,
-                                       device="cuda")
-            lf.run(su[:rows], precision=self.precision, normalize=False, out=scratch)
+        """Size the streamed contraction's workspace for chunks of up to `rows` rows per
+        member (and a ring wrap), so that no timed region allocates."""
+        saved = list(self.rows)
+        self.rows = [0] * len(self.lfs)
+        jobs = self._jobs(lambda i: min(rows, self.egms[i].shape[0]))
+        self.stream.reserve(jobs + jobs)
+        self.rows = saved
 
     def normalized_egm(self, i):
         from pyheartsim_amd import _lib

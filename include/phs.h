@@ -247,6 +247,27 @@ int phs_egm_create(int32_t n_voxel, int32_t n_electrode, const double* voxel,
 int phs_egm_run(phs_egm* handle, const float* samples, int64_t n_time, int64_t sample_stride,
                 float* egm, int32_t precision, void* stream);
 int phs_egm_destroy(phs_egm* handle);
+/* One job of a streamed contraction: the n_time (1..16) newest sample rows of one simulation,
+ * samples device float [n_time][sample_stride] (first n_voxel columns used, in the order the
+ * lead field was built for), egm device float [n_time][n_electrode] receiving those rows of
+ * the (un-normalised) electrogram. */
+typedef struct phs_egm_job {
+  phs_egm* lead_field;
+  const float* samples;
+  int64_t n_time;
+  int64_t sample_stride;
+  float* egm;
+} phs_egm_job;
+/* Replaces the per-batch calls of simulation/unipolar_electrogram_gpu.py:124-146 for the rows
+ * that a batch of simulations produced since the last call: every job's rows in one launch
+ * (per 64 jobs) of BF16 hi/lo split MFMA products with FP32 accumulation, K split into
+ * 1024-voxel chunks summed in chunk order, so a job's result does not depend on the other
+ * jobs.  workspace: caller-owned device memory of at least phs_egm_stream_workspace bytes,
+ * reused by every call (calls on one workspace must be ordered on one stream).  No
+ * allocation; stream-ordered. */
+int phs_egm_stream_workspace(const phs_egm_job* jobs, int32_t n_jobs, int64_t* bytes);
+int phs_egm_stream(const phs_egm_job* jobs, int32_t n_jobs, void* workspace,
+                   int64_t workspace_bytes, void* stream);
 /* Scale egm [n_time][n_electrode] in place so the mean peak-to-peak is `typical`
  * (simulation/unipolar_electrogram_equation_parts.py:118-124). */
 int phs_egm_normalize(float* egm, int64_t n_time, int32_t n_electrode, double typical,

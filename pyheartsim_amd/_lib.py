@@ -70,6 +70,16 @@ class ArenaConfig(C.Structure):
     ]
 
 
+class EgmJob(C.Structure):
+    _fields_ = [
+        ("lead_field", C.c_void_p),
+        ("samples", C.c_void_p),
+        ("n_time", C.c_int64),
+        ("sample_stride", C.c_int64),
+        ("egm", C.c_void_p),
+    ]
+
+
 _P = C.c_void_p
 _I32 = C.c_int32
 _I64 = C.c_int64
@@ -111,6 +121,8 @@ SIGNATURES = {
     "phs_egm_create": (C.c_int, [_I32, _I32, _P, _P, _P, _P, _P, _I32, _D, _P, C.POINTER(_P), _P]),
     "phs_egm_run": (C.c_int, [_P, _P, _I64, _I64, _P, _I32, _P]),
     "phs_egm_destroy": (C.c_int, [_P]),
+    "phs_egm_stream_workspace": (C.c_int, [_P, _I32, _PI64]),
+    "phs_egm_stream": (C.c_int, [_P, _I32, _P, _I64, _P]),
     "phs_egm_normalize": (C.c_int, [_P, _I64, _I32, _D, _P]),
     "phs_egm_electrode_lat": (C.c_int, [_P, _I64, _I32, _P, _P]),
     "phs_conduction_velocity": (C.c_int, [_P, _P, _I32, _D, _P, _P, _P, _P]),

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "phs_internal.h"
@@ -362,6 +363,131 @@ __global__ void egm_reduce(const float* __restrict__ ws, int splits, int64_t T, 
   out[i] = s;
 }
 
+// ---------------------------------------------------------------- streamed contraction
+// A few new sample rows (T <= 16) of many simulations at once, e.g. the rows a batch of
+// simulations produced during the last few hundred steps.  One launch covers up to
+// kStreamJobs jobs (a job: one lead field, its T new rows and their output rows); work items are
+// (job, 1024-voxel chunk, 64-electrode tile), one wavefront each.  Per 64-deep K stage the
+// wavefront runs the BF16 hi/lo split products of one 16-row time block against four
+// 16-electrode blocks on v_mfma_f32_16x16x32_bf16, with the lead-field operands loaded
+// straight from the K-stage-major tiles (each lane's 8 k-values of one electrode are 16
+// contiguous bytes) and the samples split into hi/lo in registers: no LDS and no barriers.
+// At T = 1 the work is bound by the lead-field stream (4 bytes per electrode and voxel), not
+// by the MFMAs.  Each item writes its partial (T x 64) block to the workspace; a second kernel
+// sums the chunks of each output element in chunk order, so the result depends only on the
+// job itself, not on the other jobs of the launch.
+constexpr int kStreamJobs = 64;
+constexpr int kStreamRows = 16;    // time rows of one job (one MFMA row block)
+constexpr int kStreamStages = 16;  // 64-deep K stages per work item (1024 voxels)
+constexpr int kStreamWaves = 4;    // wavefronts (independent items) per workgroup
+
+struct StreamJob {
+  const unsigned short* Wt;
+  const float* S;
+  float* out;
+  int64_t stride;
+  int32_t T, N, Epad, E;
+  int32_t item0;  // first work item of the job
+  int32_t out0;   // first output element of the job (reduction)
+};
+struct StreamTable {
+  int32_t n;
+  StreamJob j[kStreamJobs];
+};
+
+__device__ __forceinline__ int stream_chunks(int N) {
+  return ((N + kK2 - 1) / kK2 + kStreamStages - 1) / kStreamStages;
+}
+
+__global__ void __launch_bounds__(64 * kStreamWaves) egm_stream_k(const StreamTable tab,
+                                                                   float* __restrict__ ws,
+                                                                   int n_items) {
+  const int item = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kStreamWaves +
+                                                        (threadIdx.x >> 6)));
+  if (item >= n_items) return;
+  int j = 0;
+  for (int q = 1; q < tab.n; ++q)
+    if (tab.j[q].item0 <= item) j = q;
+  const StreamJob& J = tab.j[j];
+  const int lane = threadIdx.x & 63, r16 = lane & 15, q = lane >> 4;
+  const int etiles = J.Epad >> 6;
+  const int li = item - J.item0;
+  const int et = li % etiles, ch = li / etiles;
+  const int nst = (J.N + kK2 - 1) / kK2;
+  const int s0 = ch * kStreamStages, s1 = min(nst, s0 + kStreamStages);
+  const int t = r16 < J.T ? r16 : J.T - 1;  // rows >= T repeat row T - 1 (never written)
+  const float* arow = J.S + (int64_t)t * J.stride;
+  const size_t plane = (size_t)J.Epad * kK2;
+  f32x4 acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int s = s0; s < s1; ++s) {
+    const unsigned short* tile = J.Wt + (size_t)s * 2 * plane + (size_t)(et * 64 + r16) * kK2;
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2) {
+      const int k = s * kK2 + 32 * s2 + 8 * q;
+      float a[8];
+      if (k + 8 <= J.N) {
+        float4 x0, x1;
+        __builtin_memcpy(&x0, arow + k, 16);
+        __builtin_memcpy(&x1, arow + k + 4, 16);
+        a[0] = x0.x; a[1] = x0.y; a[2] = x0.z; a[3] = x0.w;
+        a[4] = x1.x; a[5] = x1.y; a[6] = x1.z; a[7] = x1.w;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a[i] = k + i < J.N ? arow[k + i] : 0.0f;
+      }
+      unsigned h[4], l[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) split_pair(a[2 * i], a[2 * i + 1], h[i], l[i]);
+      const bf16x8 ah = __builtin_bit_cast(bf16x8, uint4{h[0], h[1], h[2], h[3]});
+      const bf16x8 al = __builtin_bit_cast(bf16x8, uint4{l[0], l[1], l[2], l[3]});
+      uint4 bh[4], bl[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const unsigned short* p = tile + (size_t)(16 * c) * kK2 + 32 * s2 + 8 * q;
+        bh[c] = *(const uint4*)p;
+        bl[c] = *(const uint4*)(p + plane);
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const bf16x8 h8 = __builtin_bit_cast(bf16x8, bh[c]);
+        const bf16x8 l8 = __builtin_bit_cast(bf16x8, bl[c]);
+        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, h8, acc[c], 0, 0, 0);
+        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, l8, acc[c], 0, 0, 0);
+        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, h8, acc[c], 0, 0, 0);
+      }
+    }
+  }
+  // result lane: electrode 16 c + r16 of the tile, time rows 4 q .. 4 q + 3
+  float* w = ws + (size_t)item * kStreamRows * 64;
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      const int row = 4 * q + jj;
+      if (row < J.T) w[row * 64 + 16 * c + r16] = acc[c][jj];
+    }
+}
+
+__global__ void egm_stream_reduce_k(const StreamTable tab, const float* __restrict__ ws,
+                                    int n_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  int j = 0;
+  for (int q = 1; q < tab.n; ++q)
+    if (tab.j[q].out0 <= i) j = q;
+  const StreamJob& J = tab.j[j];
+  const int li = i - J.out0;
+  const int t = li / J.E, e = li % J.E;
+  const int etiles = J.Epad >> 6;
+  const int chunks = stream_chunks(J.N);
+  const float* p = ws + ((size_t)(J.item0 + (e >> 6)) * kStreamRows + t) * 64 + (e & 63);
+  float sum = 0.0f;
+  for (int c = 0; c < chunks; ++c) sum += p[(size_t)c * etiles * kStreamRows * 64];
+  J.out[li] = sum;
+}
+
 constexpr int kNormThreads = 1024;
 constexpr int kNormMaxE = 4096;
 
@@ -619,6 +745,81 @@ int phs_egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, flo
   const int64_t n = T * g->E;
   egm_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g->ws, splits, T, g->Epad, g->E, egm);
   PHS_HIP_CHECK(hipGetLastError());
+  return PHS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int host_stream_chunks(int N) { return ((N + kK2 - 1) / kK2 + kStreamStages - 1) / kStreamStages; }
+
+// Items of the launch groups (kStreamJobs consecutive jobs each); -1 with the error set.
+int64_t stream_check(const phs_egm_job* jobs, int32_t n_jobs, int64_t* max_items) {
+  *max_items = 0;
+  for (int32_t g0 = 0; g0 < n_jobs; g0 += kStreamJobs) {
+    int64_t items = 0;
+    for (int32_t i = g0; i < std::min(n_jobs, g0 + kStreamJobs); ++i) {
+      const phs_egm_job& J = jobs[i];
+      if (!J.lead_field || !J.samples || !J.egm || J.n_time < 1 || J.n_time > kStreamRows ||
+          J.sample_stride < J.lead_field->N)
+        return phs::fail(PHS_ERR_ARG, "invalid streamed electrogram job (n_time must be 1..16, "
+                                      "sample_stride >= n_voxel)");
+      items += (int64_t)host_stream_chunks(J.lead_field->N) * (J.lead_field->Epad / 64);
+    }
+    *max_items = std::max(*max_items, items);
+  }
+  if (*max_items >= (1LL << 30)) return phs::fail(PHS_ERR_ARG, "too many streamed work items");
+  return PHS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phs_egm_stream_workspace(const phs_egm_job* jobs, int32_t n_jobs, int64_t* bytes) {
+  if (!jobs || n_jobs <= 0 || !bytes) return phs::fail(PHS_ERR_ARG, "invalid arguments");
+  int64_t items = 0;
+  if (int rc = stream_check(jobs, n_jobs, &items)) return rc;
+  *bytes = items * kStreamRows * 64 * (int64_t)sizeof(float);
+  return PHS_OK;
+}
+
+int phs_egm_stream(const phs_egm_job* jobs, int32_t n_jobs, void* workspace,
+                   int64_t workspace_bytes, void* stream) {
+  if (!jobs || n_jobs <= 0 || !workspace) return phs::fail(PHS_ERR_ARG, "invalid arguments");
+  int64_t items = 0;
+  if (int rc = stream_check(jobs, n_jobs, &items)) return rc;
+  if (workspace_bytes < items * kStreamRows * 64 * (int64_t)sizeof(float))
+    return phs::fail(PHS_ERR_ARG, "workspace smaller than phs_egm_stream_workspace");
+  hipStream_t s = (hipStream_t)stream;
+  float* ws = (float*)workspace;
+  for (int32_t g0 = 0; g0 < n_jobs; g0 += kStreamJobs) {
+    StreamTable tab;
+    std::memset(&tab, 0, sizeof tab);
+    int32_t n_items = 0, n_out = 0;
+    for (int32_t i = g0; i < std::min(n_jobs, g0 + kStreamJobs); ++i) {
+      const phs_egm_job& J = jobs[i];
+      const phs_egm* g = J.lead_field;
+      StreamJob& d = tab.j[tab.n++];
+      d.Wt = g->Wt;
+      d.S = J.samples;
+      d.out = J.egm;
+      d.stride = J.sample_stride;
+      d.T = (int32_t)J.n_time;
+      d.N = g->N;
+      d.Epad = g->Epad;
+      d.E = g->E;
+      d.item0 = n_items;
+      d.out0 = n_out;
+      n_items += host_stream_chunks(g->N) * (g->Epad / 64);
+      n_out += d.T * g->E;
+    }
+    egm_stream_k<<<(n_items + kStreamWaves - 1) / kStreamWaves, 64 * kStreamWaves, 0, s>>>(
+        tab, ws, n_items);
+    egm_stream_reduce_k<<<(n_out + 255) / 256, 256, 0, s>>>(tab, ws, n_out);
+    PHS_HIP_CHECK(hipGetLastError());
+  }
   return PHS_OK;
 }
 

@@ -82,6 +82,54 @@ class LeadField:
             pass
 
 
+class StreamContraction:
+    """Contraction of a few new sample rows (1..16 per job) of many simulations in one launch
+    (phs_egm_stream): each job is (LeadField, samples (T x stride) CUDA view, out (T x E)
+    contiguous CUDA view).  The workspace is a device buffer kept between calls; `reserve`
+    sizes it ahead of time so that later calls allocate nothing."""
+
+    def __init__(self):
+        self.lib = _lib.load()
+        self.ws = None
+
+    def _table(self, jobs):
+        arr = (_lib.EgmJob * len(jobs))()
+        for d, (lf, samples, out) in zip(arr, jobs):
+            if samples.dim() != 2 or samples.stride(1) != 1:
+                raise ValueError("samples must be a (T x stride) row-major view")
+            T, stride = samples.shape[0], samples.stride(0)
+            if out.shape[0] != T or out.shape[1] != lf.E or not out.is_contiguous():
+                raise ValueError("out must be a contiguous (T x E) view")
+            d.lead_field = lf.handle
+            d.samples = _lib.ptr(samples)
+            d.n_time = T
+            d.sample_stride = stride
+            d.egm = _lib.ptr(out)
+        return arr
+
+    def _need(self, arr) -> int:
+        b = C.c_int64()
+        _lib.check(self.lib.phs_egm_stream_workspace(arr, len(arr), C.byref(b)),
+                   "phs_egm_stream_workspace")
+        return b.value
+
+    def reserve(self, jobs):
+        import torch
+        need = self._need(self._table(jobs))
+        if self.ws is None or self.ws.numel() * 4 < need:
+            self.ws = torch.empty(max(need // 4, 1), dtype=torch.float32, device="cuda")
+
+    def run(self, jobs, stream=None):
+        if not jobs:
+            return
+        arr = self._table(jobs)
+        need = self._need(arr)
+        if self.ws is None or self.ws.numel() * 4 < need:
+            self.reserve(jobs)
+        _lib.check(self.lib.phs_egm_stream(arr, len(arr), _lib.ptr(self.ws), self.ws.numel() * 4,
+                                           _lib.stream_ptr(stream)), "phs_egm_stream")
+
+
 def compute(electrode_xyz, voxel, D0, c_voxel, action_potential, Delta, neighbor_id_2d,
             batch_size=50, *, precision: str = "bf16x3", normalize: bool = True,
             device_out: bool = False):
