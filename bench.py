@@ -145,21 +145,43 @@ def step_bytes(cnt, sweeps, guess=2, samples_per_step=0.0):
     return total, per
 
 
-def mid_sweep_roofline(cnt, prof, sweeps, ms_per_step, guess=2):
-    """The roofline record of the dominant kernel (the middle Chebyshev sweep: launched
-    m - 3 times per step, the largest share of the step) from its HIP-event duration."""
+def step_roofline(cnt, timing, sweeps, ms_per_step, guess=2, isolated=None):
+    """Roofline of the dominant kernel of an arena step.  `timing` is [(kind, ms), ...]: the
+    launch durations of one step measured in place inside a replayed graph of the timed region
+    (HIP events between the launches, on the arena's stream).  The dominant kernel is the kind
+    with the largest total time per step (the intermediate Chebyshev sweep, launched m - 3
+    times); achieved = its algorithmic bytes per launch / its average launch duration."""
     total, per = step_bytes(cnt, sweeps, guess)
-    achieved = per["mid"] / (prof["sweep"] * 1e-3) / 1e9
-    share = (sweeps - 3) * prof["sweep"] / ms_per_step if ms_per_step > 0 else None
+    key = {"rhs": "rhs", "first_sweep": "first", "mid_sweep": "mid", "last_sweep": "last"}
+    kinds = {}
+    for kind, ms in timing:
+        k = kinds.setdefault(kind, [0, 0.0])
+        k[0] += 1
+        k[1] += ms
+    table = {}
+    for kind, (n, ms) in kinds.items():
+        avg = ms / n
+        b = per.get(key.get(kind, ""), None)
+        table[kind] = dict(launches=n, avg_ms=avg, bytes_per_launch=b,
+                           gbs=(b / (avg * 1e-3) / 1e9) if b else None,
+                           frac=(b / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS) if b else None)
+    dom = max(kinds, key=lambda k: kinds[k][1])
+    d = table[dom]
+    in_step = sum(ms for _, ms in timing)
     return {
-        "bound": "hbm", "kernel": "k_sweep (middle Chebyshev sweep, FP32)",
-        "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(cnt),
-        "bytes_per_launch": per["mid"], "avg_launch_ms": prof["sweep"],
-        "launches_per_step": sweeps - 3, "share_of_step": share,
-        "kernels_ms": prof,
+        "bound": "hbm", "kernel": f"{dom} (k_sweep / k_rhs of the bricked FP32 arena)",
+        "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"],
+        "traffic": profiled_traffic(cnt) if dom == "mid_sweep" else None,
+        "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_ms"],
+        "launches_per_step": d["launches"],
+        "share_of_step": kinds[dom][1] / in_step if in_step > 0 else None,
+        "timing": "HIP events around each launch of the last step of the timed region's "
+                  "last graph replay",
+        "kernels": table,
+        "step_kernels_ms": in_step,
         "step_algorithmic_bytes": total,
         "step_hbm_frac": total / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "isolated_kernels_ms": isolated,
     }
 
 
@@ -244,7 +266,8 @@ class Chain:
         """Electrogram rows of every sample produced since the last call: one streamed
         contraction over all simulations (phs_egm_stream)."""
         jobs = self._jobs(self.produced)
-        self.stream.run(jobs)
+        if jobs:
+            self.stream.run(jobs)
         for i in range(len(self.lfs)):
             self.rows[i] = max(self.rows[i], self.produced(i))
 
@@ -259,6 +282,13 @@ class Chain:
             done += c
             if self.egm:
                 self.contract()
+
+    def advance_marked(self):
+        """One step with HIP events around each launch, then the contraction of its rows."""
+        self.arena.step_marked()
+        self.steps += 1
+        if self.egm:
+            self.contract()
 
     def warm_egm(self, rows):
         """Size the streamed contraction's workspace for chunks of up to `rows` rows per
@@ -284,17 +314,21 @@ class Chain:
 
 def timed_chain(torch, chain, steps, warmup, chunk):
     """W untimed warm-up steps, then exactly `steps` steps timed between device
-    synchronisations; graphs of the timed chunk length are captured beforehand."""
+    synchronisations: steps - 1 replayed from graphs of up to `chunk` steps (captured
+    beforehand) and the last one launched eagerly with HIP events around each of its launches
+    (read afterwards with arena.step_timing())."""
     if steps <= 0:
         raise SystemExit("--steps must be > 0")
-    c = min(chunk, steps)
+    c = max(1, min(chunk, steps - 1))
     chain.arena.prepare_graph(c)
     if warmup > 0:
         chain.advance(warmup, min(chunk, warmup))
         chain.arena.prepare_graph(c)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    chain.advance(steps, c)
+    if steps > 1:
+        chain.advance(steps - 1, c)
+    chain.advance_marked()
     torch.cuda.synchronize()
     return time.perf_counter() - t0
 
@@ -453,6 +487,19 @@ def run_batched(args, torch, runtime, pre):
     value = cnt["voxels"] * K / t
     ms_step = 1e3 * t / K
     log(f"batched: {value:.4g} voxel-steps/s, {ms_step:.4f} ms/step over {K} steps")
+    timing = chain.arena.step_timing()
+    log(f"step launches (ms): {timing}")
+    if args.no_validate:
+        prof = chain.arena.profile(50)
+        cnt_line = dict(cnt)
+        chain.close()
+        return {"metric": METRIC, "value": value, "unit": "voxel-steps/s", "n_gpus": 1,
+                "steps": K, "warmup": W, "ms_per_step": ms_step, "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                "config": {"workload": "batched (timed region only, not validated)",
+                           "voxels": cnt_line["voxels"], "sweeps": sweeps},
+                "roofline": step_roofline(cnt, timing, sweeps, ms_step, args.guess, prof),
+                "cpu_baseline": None}
     # sustained rate: the rest of the 1000 ms of every member, electrograms included
     n_total = members[0]["prep"].n_steps
     rest = n_total - chain.steps
@@ -465,7 +512,7 @@ def run_batched(args, torch, runtime, pre):
                      ms_per_step=1e3 * t_rest / rest, steps=rest)
     log(f"sustained: {sustained}")
     prof = chain.arena.profile(50)
-    roof = mid_sweep_roofline(cnt, prof, sweeps, ms_step, args.guess)
+    roof = step_roofline(cnt, timing, sweeps, ms_step, args.guess, prof)
     roof["sustained_step_hbm_frac"] = (roof["step_algorithmic_bytes"] /
                                        (sustained["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
     log(f"kernels: {prof}")
@@ -547,6 +594,7 @@ def run_shell(args, torch, runtime, pre):
     nspm = members[0]["prep"].steps_per_sample
     chain.warm_egm(math.ceil(args.graph_steps / nspm) + 1)
     t = timed_chain(torch, chain, args.steps, args.warmup, args.graph_steps)
+    timing = chain.arena.step_timing()
     N = members[0]["prep"].n_voxel
     value = N * args.steps / t
     chain.advance(members[0]["prep"].n_steps - chain.steps, args.graph_steps)
@@ -566,7 +614,7 @@ def run_shell(args, torch, runtime, pre):
                                f"64-electrode electrogram, alone", "n_voxel": N,
                    "sweeps": sweeps, "guess": args.guess},
         "speedup_vs_plain_fp64": value / rate, "validate": rec, "legs": legs,
-        "roofline": mid_sweep_roofline(cnt, prof, sweeps, 1e3 * t / args.steps, args.guess),
+        "roofline": step_roofline(cnt, timing, sweeps, 1e3 * t / args.steps, args.guess, prof),
         "cpu_baseline": None,
     }
 
@@ -735,6 +783,9 @@ def parser():
     ap.add_argument("--workers", type=int, default=0,
                     help="host processes for building the workloads (0 = the host threads "
                          "this job may use, at most 16)")
+    ap.add_argument("--no-validate", action="store_true",
+                    help="batched workload: stop after the timed region (profiler runs); the "
+                         "line then carries no validation, serve or CPU figures")
     ap.add_argument("--only-headline", action="store_true",
                     help="batched workload only: skip the shell legs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",

@@ -110,12 +110,22 @@ int phs_device_count(int32_t* count);
  * (*stable) and the arena's automatic count (*automatic, may be NULL): the fewest stable
  * sweeps whose error bound at the stiffest mode, lambda_max * max|P|, is <= 1/4 and under
  * which every stiff mode (lambda >= 2) shrinks by e within 40 steps (amplification over
- * [2, lambda_max] <= 1 - 1/40).
+ * [2, lambda_max] <= 1 - 1/40) and whose bound max|P| itself is <= 0.03 (the accuracy
+ * target at fine time steps; see phs_auto_sweeps for coarse ones).
  * No counterpart in the reference, whose BiCGSTAB solve runs to a tolerance. */
 int phs_stable_sweeps(double lambda_max, int32_t guess, int32_t refine, int32_t refine_sweeps,
                       int32_t* stable, int32_t* automatic);
 int phs_step_amplification(double lambda_max, int32_t guess, int32_t sweeps, int32_t refine,
                            int32_t refine_sweeps, double* amplification);
+/* Host only.  The arena's automatic sweep count for one simulation (model and params as in
+ * phs_sim_desc, dt its step): the automatic count of phs_stable_sweeps, raised where needed so
+ * that the Chebyshev error bound (times the refinement passes' bounds) stays below
+ * 0.03 min(1, (0.025 / s)^2), s = dt times the cell model's steepest upstroke rate
+ * (*resolution, may be NULL): coarse time steps leave the warm start less to predict.
+ * No counterpart in the reference, whose solver runs to a tolerance. */
+int phs_auto_sweeps(double lambda_max, int32_t guess, int32_t refine, int32_t refine_sweeps,
+                    int32_t model, const double* params, double dt, int32_t* sweeps,
+                    double* resolution);
 
 /* Host only (no device work): the Morton brick plan of a voxel set for brick shape
  * brick[3] (bx*by*bz = 64).  perm [n_voxel] receives the compact -> caller order,
@@ -190,6 +200,15 @@ int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream);
  * steps, for both launch parities, so that a later replay of that length does no capture.
  * Host only; the graphs stay valid until the set of resident simulations changes. */
 int phs_arena_graph_prepare(phs_arena* a, int32_t n_steps);
+/* One eager step (as phs_arena_step with n_steps = 1) whose launches are each bracketed by
+ * HIP events recorded on `stream`; phs_arena_step_timing then gives the duration of each
+ * launch of that step measured in place, in launch order: *n = launches, and for i <
+ * min(*n, cap) ms[i] the duration (ms) and kinds[i] the kernel: 0 right-hand side, 1 first
+ * sweep, 2 intermediate sweep, 3 last sweep (activation times, samples, next reaction),
+ * 4 refinement residual, 5 refinement sweep.  phs_arena_step_timing waits for the marked step
+ * to complete.  No counterpart in the reference (measurement only). */
+int phs_arena_step_marked(phs_arena* a, void* stream);
+int phs_arena_step_timing(phs_arena* a, int32_t cap, double* ms, int32_t* kinds, int32_t* n);
 /* Host-side view of a slot: steps done, voxels, bricks, first compact index. */
 int phs_arena_slot_info(phs_arena* a, int32_t slot, int64_t* steps_done, int32_t* n_voxel,
                         int32_t* n_bricks, int64_t* voxel_offset);

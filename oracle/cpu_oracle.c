@@ -139,6 +139,10 @@ static int bicgstab(const csr* A, const double* b, double* x, double rtol, doubl
   double *r = work, *rt = work + n, *p = work + 2 * (size_t)n, *v = work + 3 * (size_t)n,
          *s = work + 4 * (size_t)n, *t = work + 5 * (size_t)n;
   const double bn = sqrt(dot(b, b, n));
+  if (bn == 0.0) {  /* scipy returns the exact solution x = 0 for b = 0 */
+    memset(x, 0, sizeof(double) * n);
+    return 0;
+  }
   const double tol = fmax(atol, rtol * bn);
   spmv(A, x, v);
 #pragma omp parallel for

@@ -117,17 +117,31 @@ static double cheb_bound(double lmax, int m) {
   return 2.0 * qm / (1.0 + qm * qm);
 }
 
-int auto_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax) {
+int auto_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax,
+                double resolution) {
   const int k0 = stable_sweeps(lmax, guess, refine, refine_sweeps, kmax);
   if (k0 < 0) return -1;
   const double tail = std::pow(cheb_bound(lmax, refine_sweeps), refine);
   const double stiff_max = 1.0 - 1.0 / kStiffSteps;
+  const double r = resolution > kFineResolution ? kFineResolution / resolution : 1.0;
+  const double accuracy = kAccuracy * r * r;
   for (int k = k0; k <= kmax; ++k)
     if (lmax * cheb_bound(lmax, k) * tail <= 0.25 &&
+        (lmax <= 2.0 || cheb_bound(lmax, k) * tail <= accuracy) &&
         (lmax <= 2.0 ||
          step_amplification(lmax, guess, k, refine, refine_sweeps, 2.0) <= stiff_max))
       return k;
   return kmax;
+}
+
+double step_resolution(int model, const double* params, double dt) {
+  if (model == 0) return dt * (4.0 / 27.0) / params[2];
+  double best = 0.0;
+  for (int i = 0; i <= 1000; ++i) {
+    const double u = i / 1000.0;
+    best = std::max(best, params[0] * u * (u - params[1]) * (1.0 - u));
+  }
+  return dt * best;
 }
 
 int chebyshev_sweeps(double lmin, double lmax, double tol) {
@@ -157,6 +171,20 @@ int phs_stable_sweeps(double lambda_max, int32_t guess, int32_t refine, int32_t 
   *stable = phs::stable_sweeps(lambda_max, guess, refine, rs, 64);
   if (automatic) *automatic = phs::auto_sweeps(lambda_max, guess, refine, rs, 64);
   return *stable < 0 ? phs::fail(PHS_ERR_UNSUPPORTED, "no stable sweep count <= 64") : PHS_OK;
+}
+
+int phs_auto_sweeps(double lambda_max, int32_t guess, int32_t refine, int32_t refine_sweeps,
+                    int32_t model, const double* params, double dt, int32_t* sweeps,
+                    double* resolution) {
+  if (!sweeps || !params || (model != 0 && model != 1) || !(dt > 0))
+    return phs::fail(PHS_ERR_ARG, "invalid arguments");
+  if (guess < 0 || guess > 2 || refine < 0 || refine > 4 ||
+      (refine > 0 && (refine_sweeps < 2 || refine_sweeps > 64)) || !(lambda_max >= 1.0))
+    return phs::fail(PHS_ERR_ARG, "invalid stability query");
+  const double s = phs::step_resolution(model, params, dt);
+  if (resolution) *resolution = s;
+  *sweeps = phs::auto_sweeps(lambda_max, guess, refine, refine > 0 ? refine_sweeps : 0, 64, s);
+  return *sweeps < 0 ? phs::fail(PHS_ERR_UNSUPPORTED, "no stable sweep count <= 64") : PHS_OK;
 }
 
 int phs_step_amplification(double lambda_max, int32_t guess, int32_t sweeps, int32_t refine,

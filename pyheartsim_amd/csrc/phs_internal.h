@@ -45,8 +45,22 @@ int stable_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kma
 //      factor e within kStiffSteps steps: amplification over [2, lmax] <= 1 - 1/kStiffSteps.
 //      The smooth modes (lam < 2) keep CN's physical factor close to 1.
 // (clamped to kmax; -1 when no count is stable).
+//  (3) the solve is accurate enough for the step's time resolution: with s = dt times the cell
+//      model's steepest upstroke rate (step_resolution), the Chebyshev bound (times the
+//      refinement passes' bounds) is <= kAccuracy * min(1, (kFineResolution / s)^2).  A
+//      warm-started solve corrects only what the previous steps did not predict, which grows
+//      like s^2 for the extrapolated warm start; the constants are those of the 0.4 mm shell at
+//      dt = 0.05 ms (s = 0.025, 6 sweeps), which meets the activation-time and electrogram bars
+//      over 40 000 steps, so steps that coarse or finer keep the first two conditions only.
 constexpr double kStiffSteps = 40.0;
-int auto_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax);
+constexpr double kAccuracy = 0.03;
+constexpr double kFineResolution = 0.025;
+int auto_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax,
+                double resolution = 0.0);
+// dt times the steepest upstroke rate of the cell model at rest-to-peak (h = 1 for
+// Mitchell-Schaeffer, h = 0 for Aliev-Panfilov): MS (4/27) / tau_in, AP max_u k u (u - a) (1 - u).
+// params as in phs_sim_desc.
+double step_resolution(int model, const double* params, double dt);
 
 }  // namespace phs
 

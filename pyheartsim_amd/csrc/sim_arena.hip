@@ -128,9 +128,29 @@ struct phs_arena {
   int32_t par = 0;         // launch parity of the next step (flips with every launched step)
   int32_t sweeps_eff = 2;  // sweeps of the first CN solve in the launched sequence
   int32_t flav = 0;        // kernel flavour (FLAV_*) the resident simulations allow
+  // Launch timing of one eager step: while `marking`, every launch of the step is followed
+  // by an event record (event 0 before the first launch), and kinds[i] labels launch i.
+  static constexpr int kMaxMarks = 2 + kMaxSweeps + 4 * (kMaxSweeps + 1);
+  hipEvent_t marks[kMaxMarks] = {};
+  int32_t kinds[kMaxMarks] = {};
+  int32_t n_marks = 0;     // events recorded by the marked step (launches + 1)
+  bool marking = false;
+  bool marks_valid = false;  // a marked step has been launched
 };
 
 namespace {
+
+enum { KIND_RHS = 0, KIND_FIRST = 1, KIND_MID = 2, KIND_LAST = 3, KIND_RESIDUAL = 4,
+       KIND_REFINE = 5 };
+
+// After a launch of kind `kind` in a marked step: record the event that closes it (event 0
+// is recorded before the step's first launch).
+void mark(phs_arena* a, hipStream_t s, int kind) {
+  if (!a->marking || a->n_marks < 1 || a->n_marks >= phs_arena::kMaxMarks) return;
+  a->kinds[a->n_marks - 1] = kind;
+  hipEventRecord(a->marks[a->n_marks], s);
+  ++a->n_marks;
+}
 
 ArenaDev dev_view(const phs_arena* a) {
   ArenaDev A;
@@ -214,6 +234,7 @@ struct Step {
     const phs_arena_config& c = a->cfg;
     // the step's reaction already ran in the previous step's last sweep (or at add)
     k_rhs<BX, BY, BZ, NB, GUESS, FLAV><<<grid, kBlock, 0, s>>>(A);
+    mark(a, s, KIND_RHS);
     float* bufs[2] = {a->DA, a->DB};
     // One Chebyshev solve of m sweeps on the right-hand side held in R0; per_slot: each
     // simulation runs its own count (<= m) of the first solve.
@@ -225,6 +246,8 @@ struct Step {
         const int prev_kind = k == 1 ? 0 : (k == 2 ? 1 : 2);
         sweep<FLAV>(k < m - 1 ? MODE_MID : last_mode, grid, s, A, k, K, cur, prev, prev_kind,
                     bufs[(k + 1) % 2]);
+        mark(a, s, !per_slot ? KIND_REFINE : k == 1 ? KIND_FIRST : k < m - 1 ? KIND_MID
+                                                                             : KIND_LAST);
       }
     };
     if (c.refine <= 0) {
@@ -233,6 +256,7 @@ struct Step {
       solve(a->sweeps_eff, MODE_DACC_SET, true);
       for (int r = 0; r < c.refine; ++r) {
         k_residual<BX, BY, BZ><<<grid1, kBlock, 0, s>>>(A);
+        mark(a, s, KIND_RESIDUAL);
         solve(c.refine_sweeps, r == c.refine - 1 ? MODE_DACC_ADD_X : MODE_DACC_ADD, false);
       }
     }
@@ -618,7 +642,10 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
                     c.sweeps, h.lmax, c.guess, c.refine, need);
       return undo(PHS_ERR_ARG, msg);
     }
-    h.need_sweeps = c.sweeps > 0 ? c.sweeps : phs::auto_sweeps(h.lmax, c.guess, c.refine, rs, kMaxSweeps);
+    h.need_sweeps = c.sweeps > 0 ? c.sweeps
+                                 : phs::auto_sweeps(h.lmax, c.guess, c.refine, rs, kMaxSweeps,
+                                                    phs::step_resolution(d->model, d->params,
+                                                                         d->dt));
   }
   std::vector<phs::ChebCoef> coef(kMaxSweeps);
   SlotRec sr;
@@ -798,6 +825,41 @@ int phs_arena_step_graph(phs_arena* a, int32_t n_steps, void* stream) {
   return PHS_OK;
 }
 
+int phs_arena_step_marked(phs_arena* a, void* stream) {
+  if (!a) return phs::fail(PHS_ERR_ARG, "handle is NULL");
+  if (a->brick_high == 0) return phs::fail(PHS_ERR_STATE, "arena is empty");
+  hipStream_t s = (hipStream_t)stream;
+  if (!a->marks[0])
+    for (auto& e : a->marks) PHS_HIP_CHECK(hipEventCreate(&e));
+  a->marking = true;
+  PHS_HIP_CHECK(hipEventRecord(a->marks[0], s));
+  a->n_marks = 1;
+  launch_step(a, s);
+  a->marking = false;
+  PHS_HIP_CHECK(hipGetLastError());
+  a->marks_valid = true;
+  for (auto& h : a->sh)
+    if (h.used) h.steps_done = std::min<int64_t>(h.n_steps, h.steps_done + 1);
+  return PHS_OK;
+}
+
+int phs_arena_step_timing(phs_arena* a, int32_t cap, double* ms, int32_t* kinds, int32_t* n) {
+  if (!a || !n || cap < 0 || (cap > 0 && (!ms || !kinds)))
+    return phs::fail(PHS_ERR_ARG, "invalid arguments");
+  if (!a->marks_valid || a->n_marks < 2)
+    return phs::fail(PHS_ERR_STATE, "no marked step has been launched");
+  const int launches = a->n_marks - 1;
+  PHS_HIP_CHECK(hipEventSynchronize(a->marks[launches]));
+  for (int i = 0; i < launches && i < cap; ++i) {
+    float t = 0.0f;
+    PHS_HIP_CHECK(hipEventElapsedTime(&t, a->marks[i], a->marks[i + 1]));
+    ms[i] = (double)t;
+    kinds[i] = a->kinds[i];
+  }
+  *n = launches;
+  return PHS_OK;
+}
+
 int phs_arena_slot_info(phs_arena* a, int32_t slot, int64_t* steps_done, int32_t* n_voxel,
                         int32_t* n_bricks, int64_t* voxel_offset) {
   if (!a || slot < 0 || slot >= a->cfg.max_sims || !a->sh[slot].used)
@@ -898,6 +960,8 @@ int phs_arena_destroy(phs_arena* a) {
     if (G.graph) hipGraphDestroy(G.graph);
   }
   if (a->cap_stream) hipStreamDestroy(a->cap_stream);
+  for (auto& e : a->marks)
+    if (e) hipEventDestroy(e);
   hipFree(a->bricks);
   hipFree(a->slots);
   hipFree(a->U);

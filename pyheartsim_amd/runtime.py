@@ -237,6 +237,19 @@ def stable_sweeps(lambda_max: float, guess: int = 2, refine: int = 0, refine_swe
     return s.value, a.value
 
 
+def auto_sweeps(lambda_max: float, prep: "Prepared", guess: int = 2, refine: int = 0,
+                refine_sweeps: int = 8):
+    """(the arena's automatic sweep count for simulation `prep`, its time resolution s = dt
+    times the cell model's steepest upstroke rate) (phs_auto_sweeps)."""
+    lib = _lib.load()
+    p = np.ascontiguousarray(prep.params, dtype=np.float64)
+    m, s = C.c_int32(), C.c_double()
+    _lib.check(lib.phs_auto_sweeps(float(lambda_max), guess, refine, refine_sweeps, prep.model,
+                                   _lib.ptr(p), float(prep.dt), C.byref(m), C.byref(s)),
+               "phs_auto_sweeps")
+    return m.value, s.value
+
+
 def step_amplification(lambda_max: float, sweeps: int, guess: int = 2, refine: int = 0,
                        refine_sweeps: int = 8) -> float:
     """Largest per-step growth factor of any mode of the inexact CN step (phs_step_amplification)."""
@@ -336,6 +349,24 @@ class Arena:
         """Capture the n-step graphs of both launch parities now, so that replays of n steps
         do no capture (phs_arena_graph_prepare)."""
         _lib.check(self.lib.phs_arena_graph_prepare(self.handle, n), "phs_arena_graph_prepare")
+
+    STEP_KINDS = ("rhs", "first_sweep", "mid_sweep", "last_sweep", "residual", "refine_sweep")
+
+    def step_marked(self, stream=None):
+        """One eager step with HIP events around each launch (phs_arena_step_marked)."""
+        _lib.check(self.lib.phs_arena_step_marked(self.handle, _lib.stream_ptr(stream)),
+                   "phs_arena_step_marked")
+
+    def step_timing(self):
+        """Launch durations (ms, HIP events) of the latest marked step, in launch order:
+        [(kind, ms), ...] (phs_arena_step_timing)."""
+        cap = 512
+        ms = (C.c_double * cap)()
+        kinds = (C.c_int32 * cap)()
+        n = C.c_int32()
+        _lib.check(self.lib.phs_arena_step_timing(self.handle, cap, ms, kinds, C.byref(n)),
+                   "phs_arena_step_timing")
+        return [(self.STEP_KINDS[kinds[i]], ms[i]) for i in range(min(n.value, cap))]
 
     def steps_done(self, slot: int) -> int:
         s = C.c_int64()

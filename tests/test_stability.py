@@ -131,13 +131,18 @@ def _stiff_radius(lmax, m, guess):
     return max(mode_radius(l, lmax, m, guess) for l in np.linspace(2.0, lmax, 4097))
 
 
-def _accurate(lmax, m):
+def _bound(lmax, m):
     q = (np.sqrt(lmax) - 1.0) / (np.sqrt(lmax) + 1.0)
-    return lmax * 2.0 * q ** m / (1.0 + q ** (2 * m)) <= 0.25
+    return 2.0 * q ** m / (1.0 + q ** (2 * m))
 
 
-@pytest.mark.parametrize("lmax,guess", [(8.5, 1), (8.5, 2), (16.0, 2), (31.0, 1), (31.0, 2),
-                                        (60.0, 1)])
+def _accurate(lmax, m, resolution=0.0):
+    target = 0.03 * min(1.0, (0.025 / resolution) ** 2) if resolution > 0 else 0.03
+    return lmax * _bound(lmax, m) <= 0.25 and (lmax <= 2.0 or _bound(lmax, m) <= target)
+
+
+@pytest.mark.parametrize("lmax,guess", [(3.0, 2), (8.5, 1), (8.5, 2), (16.0, 2), (31.0, 1),
+                                        (31.0, 2), (60.0, 1)])
 def test_automatic_count_rule(lmax, guess):
     stable, auto = runtime.stable_sweeps(lmax, guess)
 
@@ -147,3 +152,26 @@ def test_automatic_count_rule(lmax, guess):
 
     assert ok(auto)
     assert not any(ok(m) for m in range(stable, auto))
+
+
+@pytest.mark.parametrize("dt,expect", [(0.05, 6), (0.5, 16)])
+def test_automatic_count_follows_time_resolution(dt, expect):
+    """The 0.4 mm shell's spectrum at dt = 0.05 ms keeps the fine-step count (6); the
+    reference's patient-mesh step dt = 0.5 ms (s = 0.25, ten times coarser) needs the
+    Chebyshev bound below 0.03 / 100: 16 sweeps at lambda_max = 13 (1 mm voxels)."""
+    from tests.helpers import small_shell
+    w = small_shell(dt=dt, t_final=2 * dt)
+    P = w["P_2d"]
+    prep = runtime.prepare(P.shape[0], P, w["geometry"], w["sim"], w["arr"])
+    lmax = 8.5 if dt == 0.05 else 13.0
+    m, s = runtime.auto_sweeps(lmax, prep, guess=2)
+    # Mitchell-Schaeffer: s = dt (4/27) / tau_in
+    assert s == pytest.approx(dt * (4.0 / 27.0) / 0.3, rel=1e-12)
+    assert m == expect
+    stable, _ = runtime.stable_sweeps(lmax, 2)
+
+    def ok(k):
+        return (k >= stable and _accurate(lmax, k, s)
+                and _stiff_radius(lmax, k, 2) <= 1.0 - 1.0 / 40.0)
+
+    assert ok(m) and not any(ok(k) for k in range(stable, m))
