@@ -266,6 +266,18 @@ int phs_egm_create(int32_t n_voxel, int32_t n_electrode, const double* voxel,
 int phs_egm_run(phs_egm* handle, const float* samples, int64_t n_time, int64_t sample_stride,
                 float* egm, int32_t precision, void* stream);
 int phs_egm_destroy(phs_egm* handle);
+/* phs_egm_run without allocation: no counterpart in the reference; for callers that cannot
+ * call hipMalloc (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  Same arguments, kernels and results as phs_egm_run (bitwise), but the split-K
+ * workspace is a library-owned static device array of phs_egm_scratch_bytes bytes (64 MiB)
+ * per device, held until the process ends, instead of a hipMalloc'd buffer; a request that
+ * needs more (splits x n_time x padded electrodes x 4 bytes) fails with PHS_ERR_ARG before
+ * any launch (contract fewer samples per call).  Nothing in the array needs to start at zero:
+ * the contraction writes every element the reduction then reads.  Every call on one device
+ * shares that array: calls that overlap on one GPU (other streams, other host threads) race. */
+int phs_egm_run_noalloc(phs_egm* handle, const float* samples, int64_t n_time,
+                        int64_t sample_stride, float* egm, int32_t precision, void* stream);
+int phs_egm_scratch_bytes(int64_t* bytes);
 /* One job of a streamed contraction: the n_time (1..16) newest sample rows of one simulation,
  * samples device float [n_time][sample_stride] (first n_voxel columns used, in the order the
  * lead field was built for), egm device float [n_time][n_electrode] receiving those rows of

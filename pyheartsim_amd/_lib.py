@@ -124,6 +124,8 @@ SIGNATURES = {
     "phs_egm_create": (C.c_int, [_I32, _I32, _P, _P, _P, _P, _P, _I32, _D, _P, C.POINTER(_P), _P]),
     "phs_egm_run": (C.c_int, [_P, _P, _I64, _I64, _P, _I32, _P]),
     "phs_egm_destroy": (C.c_int, [_P]),
+    "phs_egm_run_noalloc": (C.c_int, [_P, _P, _I64, _I64, _P, _I32, _P]),
+    "phs_egm_scratch_bytes": (C.c_int, [_PI64]),
     "phs_egm_stream_workspace": (C.c_int, [_P, _I32, _PI64]),
     "phs_egm_stream": (C.c_int, [_P, _I32, _P, _I64, _P]),
     "phs_egm_normalize": (C.c_int, [_P, _I64, _I32, _D, _P]),

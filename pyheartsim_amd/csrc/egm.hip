@@ -25,6 +25,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "phs_internal.h"
@@ -675,12 +677,18 @@ int phs_egm_create(int32_t N, int32_t E, const double* voxel, const double* elec
   return cleanup(PHS_OK, "");
 }
 
-int phs_egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, float* egm,
-                int32_t precision, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// The split-K contraction and its fixed-order reduction.  Workspace: with `fixed` (caller-
+// provided, `fixed_bytes` long) nothing is allocated and a request that needs more fails;
+// otherwise the handle's own workspace grows on demand.
+int egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, float* egm,
+            int32_t precision, hipStream_t s, float* fixed, size_t fixed_bytes) {
   if (!g || !samples || !egm || T <= 0 || stride < g->N)
     return phs::fail(PHS_ERR_ARG, "invalid electrogram run arguments");
   if (precision < 0 || precision > 2) return phs::fail(PHS_ERR_ARG, "precision must be 0, 1 or 2");
-  hipStream_t s = (hipStream_t)stream;
   const bool lds = precision != 1;
   const int tm = lds ? 32 * rw_for(g->ce) : kTM;
   const int te = lds ? 64 * g->ce : kTE;
@@ -715,23 +723,33 @@ int phs_egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, flo
   const int k_chunk = (ksteps + splits - 1) / splits * kstep;
   splits = (g->N + k_chunk - 1) / k_chunk;
   const size_t need = sizeof(float) * (size_t)splits * T * g->Epad;
-  if (need > g->ws_bytes) {
-    hipFree(g->ws);
-    g->ws = nullptr;
-    g->ws_bytes = 0;
-    PHS_TRY_ALLOC(g->ws, need);
-    g->ws_bytes = need;
+  float* ws = fixed;
+  if (fixed) {
+    if (need > fixed_bytes)
+      return phs::fail(PHS_ERR_ARG, "electrogram needs " + std::to_string(need) +
+                                        " bytes of split-K workspace, more than the " +
+                                        std::to_string(fixed_bytes) +
+                                        " of the static scratch: contract fewer samples per call");
+  } else {
+    if (need > g->ws_bytes) {
+      hipFree(g->ws);
+      g->ws = nullptr;
+      g->ws_bytes = 0;
+      PHS_TRY_ALLOC(g->ws, need);
+      g->ws_bytes = need;
+    }
+    ws = g->ws;
   }
   const int ncols = nt * splits;
   const int ncols_pad = (ncols + 7) / 8 * 8;
   const dim3 grid(mt, nt, splits);
   if (precision == 1) {
     egm_gemm_f32<<<grid, 256, 0, s>>>(samples, T, stride, g->N, g->Kpad, k_chunk, g->Wf,
-                                      g->Epad, g->ws);
+                                      g->Epad, ws);
   } else {
     auto launch = [&](auto kern) {
       kern<<<dim3(mt * ncols_pad), kGemmThreads, 0, s>>>(samples, T, stride, g->N, g->Kpad,
-                                                         k_chunk, g->Wt, g->Epad, g->ws, mt, nt,
+                                                         k_chunk, g->Wt, g->Epad, ws, mt, nt,
                                                          ncols);
     };
     const bool lo = precision == 0;
@@ -743,8 +761,58 @@ int phs_egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, flo
       lo ? launch(egm_gemm_lds<rw_for(1), 1, true>) : launch(egm_gemm_lds<rw_for(1), 1, false>);
   }
   const int64_t n = T * g->E;
-  egm_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g->ws, splits, T, g->Epad, g->E, egm);
+  egm_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ws, splits, T, g->Epad, g->E, egm);
   PHS_HIP_CHECK(hipGetLastError());
+  return PHS_OK;
+}
+
+// Split-K workspace of phs_egm_run_noalloc: 64 MiB per device, for callers that cannot
+// allocate.  In host code `scratch` names a host-side stand-in, never the device array: its
+// device address comes from hipGetSymbolAddress, once per device (scratch_on_device).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) float scratch[kScratchBytes / sizeof(float)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first call there.
+int scratch_on_device(float** ptr) {
+  int dev = 0;
+  PHS_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return phs::fail(PHS_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return phs::fail(PHS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                        hipGetErrorString(e));
+    g_scratch_dev[dev] = (float*)p;
+  }
+  *ptr = g_scratch_dev[dev];
+  return PHS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phs_egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, float* egm,
+                int32_t precision, void* stream) {
+  return egm_run(g, samples, T, stride, egm, precision, (hipStream_t)stream, nullptr, 0);
+}
+
+int phs_egm_run_noalloc(phs_egm* g, const float* samples, int64_t T, int64_t stride, float* egm,
+                        int32_t precision, void* stream) {
+  float* ws = nullptr;
+  if (int rc = scratch_on_device(&ws)) return rc;
+  return egm_run(g, samples, T, stride, egm, precision, (hipStream_t)stream, ws, kScratchBytes);
+}
+
+int phs_egm_scratch_bytes(int64_t* bytes) {
+  if (!bytes) return phs::fail(PHS_ERR_ARG, "NULL argument");
+  *bytes = (int64_t)kScratchBytes;
   return PHS_OK;
 }
 
