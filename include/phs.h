@@ -51,7 +51,8 @@ extern "C" {
 
 /* One simulation, all host arrays in the caller's (original) voxel order.
  * Mirrors what simulation/simulation_gpu.py:261-357 reads from its five arguments:
- *   model, params[5]  <- heart_model_flag, P_2d[:, 15:20] (uniform over voxels)
+ *   model, params[5]  <- heart_model_flag, P_2d[:, 15:20] (uniform over voxels; params_voxel
+ *                        when they vary)
  *   c, c_voxel        <- P_2d[:, 20] (c_voxel NULL when uniform; scar handled by vflags)
  *   dt, delta         <- simulation_parameters['dt'], geometry_data['Delta']
  *   nb                <- geometry_data['neighbor_id_2d'] columns 0-5 (row stride nb_stride)
@@ -79,6 +80,10 @@ typedef struct phs_sim_desc {
   const float* u0;         /* host [n_voxel] initial u or NULL (rest: 0) */
   const float* h0;         /* host [n_voxel] initial h or NULL (rest: MS 1, AP 0) */
   double lat_threshold;    /* activation threshold for the per-voxel LAT (0.13) */
+  const double* params_voxel; /* host [5][n_voxel] per-voxel cell parameters (parameter k of
+                              voxel i at params_voxel[k * n_voxel + i], in the order of
+                              params) or NULL (params holds the uniform values); replaces the
+                              per-voxel reads of P_2d[n, 15..19] at simulation_gpu.py:29-41 */
 } phs_sim_desc;
 
 /* Caller-owned device outputs of one simulation. */

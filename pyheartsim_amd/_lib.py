@@ -11,6 +11,8 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libphs.so"
 
+PHS_ERR_UNSUPPORTED = -4
+
 PHS_VF_SCAR = 0x01
 PHS_VF_TEMP_BLOCK = 0x02
 PHS_VF_S1 = 0x04
@@ -37,6 +39,7 @@ class SimDesc(C.Structure):
         ("u0", C.c_void_p),
         ("h0", C.c_void_p),
         ("lat_threshold", C.c_double),
+        ("params_voxel", C.c_void_p),
     ]
 
 

@@ -74,6 +74,7 @@ struct SlotRec {
   int32_t hetero;     // per-voxel alpha in the ALPHA array
   int32_t ring;       // > 0: sample s is written to row s % ring
   int32_t nsw;        // sweeps of this simulation's first CN solve (<= the launched count)
+  int32_t hparams;    // per-voxel cell parameters in the PV arrays (general kernels only)
   float p[5];       // MS: 1/tau_open, 1/tau_close, 1/tau_in, 1/tau_out, v_gate; AP: k,a,eps0,mu1,mu2
   float dt;
   float alpha;
@@ -97,6 +98,8 @@ struct ArenaDev {
   float* D1;  // CN correction of the previous step (the warm start with guess 1)
   float* D2;  // guess 2: the next step's warm start 2 D1 - (correction of the step before),
               // formed in FP32 when D1 is written
+  const float* PV[5];  // per-voxel cell parameters (same encoding as SlotRec::p) of the
+                       // simulations with hparams, or null
   int32_t n_bricks;
   int32_t flags;     // bit 0: XCD remap; bit 1: launch parity of this step (the slots' copy
                      // of the per-step values).  One word with n_bricks, so that every kernel
@@ -413,11 +416,6 @@ __device__ __forceinline__ void rk4(const float (&p)[5], float dt, int model, in
   else
     rk4_model<1>(p, dt, J, u, h, uo, ho);
 }
-__device__ __forceinline__ void rk4(const SlotRec& S, int sc, uint8_t f, float u, float h,
-                                    float& uo, float& ho) {
-  const float p[5] = {S.p[0], S.p[1], S.p[2], S.p[3], S.p[4]};
-  rk4(p, S.dt, S.model, sc, f, u, h, uo, ho);
-}
 
 // The slot values the last sweep needs to finish a voxel.  They are read with the other
 // wave-uniform values, before the kernel's first store: the stores could alias the slot
@@ -428,10 +426,11 @@ struct SlotFinish {
   float* su;
   float* sh;
   int32_t* lat;
-  int32_t n_voxel, max_beats, next_sched, model;
+  int32_t n_voxel, max_beats, next_sched, model, hparams;
   float p[5];
   float dt, lat_thr;
   __device__ __forceinline__ void load(const SlotRec& S, int par) {
+    hparams = S.hparams;
     step = S.dyn[par].step;
     n_steps = S.n_steps;
     voxel_off = S.voxel_off;
@@ -479,8 +478,12 @@ __global__ void __launch_bounds__(kBlock) k_reaction(ArenaDev A, int b_lo, int b
     if (!on[j]) continue;
     const SlotRec& S = A.slots[A.bricks[b0 + j].slot];
     if (!S.dyn[par_of(A)].live || !valid[j]) continue;
+    float p[5] = {S.p[0], S.p[1], S.p[2], S.p[3], S.p[4]};
+    if (S.hparams)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) p[k] = ld(A.PV[k], idx[j]);
     float uo, ho;
-    rk4(S, S.dyn[par_of(A)].cur_sched, f[j], u[j], h[j], uo, ho);
+    rk4(p, S.dt, S.model, S.dyn[par_of(A)].cur_sched, f[j], u[j], h[j], uo, ho);
     stv(A.UO, idx[j], uo);
     stv(A.HO, idx[j], ho);
   }
@@ -578,7 +581,8 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
 // simulation's last step).  f and hv are the voxel's flags and gate, loaded by the caller
 // together with its stencil values.
 __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotFinish& S, int idx,
-                                             float x, uint8_t f, float hv) {
+                                             float x, uint8_t f, float hv,
+                                             const float (&p)[5]) {
   const bool now = x >= S.lat_thr;
   const bool above = (f & PHS_VF_ABOVE) != 0;
   const int64_t local = (int64_t)idx - S.voxel_off;
@@ -597,7 +601,7 @@ __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotFinish
   }
   if (S.step + 1 < S.n_steps) {
     float uo, ho;
-    rk4(S.p, S.dt, S.model, S.next_sched, f, x, hv, uo, ho);
+    rk4(p, S.dt, S.model, S.next_sched, f, x, hv, uo, ho);
     A.U[idx] = uo;
     A.H[idx] = ho;
   } else {
@@ -627,6 +631,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
   float pu[NB], ph[NB], p1[NB], p2[NB];
   uint8_t pf[NB];
   SlotFinish sf[NB];
+  float pp[NB][5];  // the voxel's cell parameters (the slot's, or per voxel)
   AlphaIn ain[NB];
   // the refinement modes' accumulator (or warm start), likewise loaded up front
   double pd[NB];
@@ -659,6 +664,11 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
         pf[j] = ld(A.VF, i);
         p1[j] = GUESS >= 1 ? ld(A.D1, i) : 0.0f;
         p2[j] = GUESS == 2 ? ld(A.D2, i) : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) pp[j][k] = sf[j].p[k];
+        if (FLAV == FLAV_GENERAL && sf[j].hparams)
+#pragma unroll
+          for (int k = 0; k < 5; ++k) pp[j][k] = ld(A.PV[k], i);
       }
     }
   }
@@ -691,7 +701,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
         A.USTAR[i] = us;
         A.DACC[i] = (double)dt;
       }
-      finish_voxel(A, sf[j], i, us + dt, pf[j], ph[j]);
+      finish_voxel(A, sf[j], i, us + dt, pf[j], ph[j], pp[j]);
     } else if (MODE == MODE_DACC_SET) {
       A.DACC[i] = pd[j] + (double)dnew;
     } else if (MODE == MODE_DACC_ADD) {
@@ -702,7 +712,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       A.DACC[i] = acc;
       if (GUESS >= 1) keep_correction<GUESS>(A, i, p1[j], (float)acc);
       if (A.diag) A.USTAR[i] = us;
-      finish_voxel(A, sf[j], i, (float)((double)us + acc), pf[j], ph[j]);
+      finish_voxel(A, sf[j], i, (float)((double)us + acc), pf[j], ph[j], pp[j]);
     }
   }
 }

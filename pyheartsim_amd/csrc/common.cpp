@@ -136,11 +136,13 @@ int auto_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax,
 
 double step_resolution(int model, const double* params, double dt) {
   if (model == 0) return dt * (4.0 / 27.0) / params[2];
+  // AP: f(u) = k u (u - a) (1 - u) peaks where f'(u) = k (-3 u^2 + 2 (1 + a) u - a) = 0
+  const double k = params[0], a = params[1];
+  const double disc = (1.0 + a) * (1.0 + a) - 3.0 * a;
   double best = 0.0;
-  for (int i = 0; i <= 1000; ++i) {
-    const double u = i / 1000.0;
-    best = std::max(best, params[0] * u * (u - params[1]) * (1.0 - u));
-  }
+  for (double u : {0.0, 1.0, ((1.0 + a) + std::sqrt(std::max(disc, 0.0))) / 3.0,
+                   ((1.0 + a) - std::sqrt(std::max(disc, 0.0))) / 3.0})
+    if (u >= 0.0 && u <= 1.0) best = std::max(best, std::fabs(k * u * (u - a) * (1.0 - u)));
   return dt * best;
 }
 

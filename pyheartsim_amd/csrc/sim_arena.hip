@@ -88,6 +88,7 @@ struct SlotHost {
   int32_t need_sweeps = 2;  // sweeps this slot asks of the arena (automatic mode)
   bool needs_vf = false;
   bool hetero = false;      // per-voxel alpha
+  bool hparams = false;     // per-voxel cell parameters
   void* staging = nullptr;  // pinned upload buffer, freed after its copies complete
   hipEvent_t upload_done = nullptr;
 };
@@ -110,6 +111,7 @@ struct phs_arena {
   double* DACC = nullptr;
   uint8_t* VF = nullptr;
   double* diag_acc = nullptr;
+  float* PV[5] = {};  // per-voxel cell parameters, allocated by the first add that has them
   std::vector<SlotHost> sh;
   RangeAlloc vox, brk;
   int64_t brick_high = 0;
@@ -124,6 +126,7 @@ struct phs_arena {
     int64_t high = -1;  // brick_high captured in the kernel arguments
     int32_t sweeps = -1;
     int32_t flav = -1;
+    const void* pv = nullptr;  // PV captured in the kernel arguments
   } g[2];
   int32_t par = 0;         // launch parity of the next step (flips with every launched step)
   int32_t sweeps_eff = 2;  // sweeps of the first CN solve in the launched sequence
@@ -167,6 +170,7 @@ ArenaDev dev_view(const phs_arena* a) {
   A.ALPHA = a->ALPHA;
   A.D1 = a->D1;
   A.D2 = a->D2;
+  for (int k = 0; k < 5; ++k) A.PV[k] = a->PV[k];
   A.n_bricks = (int32_t)a->brick_high;
   A.flags = (a->cfg.xcd_remap ? 1 : 0) | (a->par << 1);
   A.diag = a->cfg.diag ? 1 : 0;
@@ -417,7 +421,7 @@ void recompute_high(phs_arena* a) {
   bool lean = true, flags = false;
   for (auto& h : a->sh)
     if (h.used) {
-      if (h.hetero || h.need_sweeps != a->sweeps_eff) lean = false;
+      if (h.hetero || h.hparams || h.need_sweeps != a->sweeps_eff) lean = false;
       flags = flags || h.needs_vf;
     }
   a->flav = !lean ? FLAV_GENERAL : (flags ? FLAV_LEAN_FLAGS : FLAV_LEAN);
@@ -529,7 +533,9 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
   const size_t sz_br = sizeof(BrickRec) * (size_t)P.n_bricks;
   const size_t sz_sl = sizeof(SlotRec);
   const size_t sz_f = 4 * (size_t)N;
-  const size_t total = sz_br + sz_sl + 4 * sz_f + (size_t)N + (size_t)d->n_steps + 64;
+  const bool hparams = d->params_voxel != nullptr;
+  const size_t total = sz_br + sz_sl + 4 * sz_f + (size_t)N + (size_t)d->n_steps + 64 +
+                       (hparams ? 5 * sz_f + 64 : 0);
   hipError_t e = hipHostMalloc(&h.staging, total, hipHostMallocDefault);
   if (e != hipSuccess) return undo(PHS_ERR_OOM, "hipHostMalloc staging");
   char* st = (char*)h.staging;
@@ -541,10 +547,16 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
   int32_t* hperm = (int32_t*)(hal + N);
   uint8_t* hvf = (uint8_t*)(hperm + N);
   uint8_t* hsched = hvf + N;
+  // per-voxel parameters (compact order), 16-byte aligned after the schedule
+  float* hpv = hparams ? (float*)(((uintptr_t)(hsched + d->n_steps) + 15) & ~(uintptr_t)15)
+                       : nullptr;
 
   bool hetero = false;
   bool needs_vf = false;
   double amax_deg = 0.0;
+  // time resolution of the step (the automatic sweep count's accuracy term): the steepest
+  // upstroke over the voxels' cell parameters
+  double resolution = phs::step_resolution(d->model, d->params, d->dt);
   const double a0 = d->dt * d->c / (2.0 * d->delta * d->delta);
   const int bd[3] = {a->cfg.brick[0], a->cfg.brick[1], a->cfg.brick[2]};
   const int stride[3] = {1, bd[0], bd[0] * bd[1]};
@@ -600,6 +612,16 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
     if (d->c_voxel && cv != d->c) hetero = true;
     const double al = d->dt * cv / (2.0 * d->delta * d->delta);
     hal[c] = (float)al;
+    if (hparams) {
+      double q[5];
+      for (int k = 0; k < 5; ++k) q[k] = d->params_voxel[(int64_t)k * N + o];
+      if (d->model == 0 && !(q[0] > 0 && q[1] > 0 && q[2] > 0 && q[3] > 0))
+        return undo(PHS_ERR_ARG, "Mitchell-Schaeffer time constants must be > 0");
+      for (int k = 0; k < 5; ++k)
+        hpv[(size_t)k * N + c] = d->model == 0 && k < 4 ? (float)(1.0 / q[k]) : (float)q[k];
+      resolution = c == 0 ? phs::step_resolution(d->model, q, d->dt)
+                          : std::max(resolution, phs::step_resolution(d->model, q, d->dt));
+    }
   }
   // Spectral bound: 1 + 2 max(alpha_n deg_n) over conducting voxels.
   for (int32_t b = 0; b < P.n_bricks; ++b) {
@@ -644,8 +666,7 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
     }
     h.need_sweeps = c.sweeps > 0 ? c.sweeps
                                  : phs::auto_sweeps(h.lmax, c.guess, c.refine, rs, kMaxSweeps,
-                                                    phs::step_resolution(d->model, d->params,
-                                                                         d->dt));
+                                                    resolution);
   }
   std::vector<phs::ChebCoef> coef(kMaxSweeps);
   SlotRec sr;
@@ -684,6 +705,8 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
   sr.hetero = hetero ? 1 : 0;
   sr.ring = out ? std::max(0, out->sample_ring) : 0;
   sr.nsw = h.need_sweeps;
+  sr.hparams = hparams ? 1 : 0;
+  h.hparams = hparams;
   if (d->model == 0) {
     sr.p[0] = (float)(1.0 / d->params[0]);
     sr.p[1] = (float)(1.0 / d->params[1]);
@@ -701,6 +724,20 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
   if ((e = hipMalloc(&h.perm_dev, 4 * (size_t)N)) ||
       (e = hipMalloc(&h.sched_dev, (size_t)std::max<int64_t>(d->n_steps, 1))))
     return undo(PHS_ERR_OOM, std::string("slot hipMalloc: ") + hipGetErrorString(e));
+  if (hparams && !a->PV[0]) {
+    // the arena's per-voxel parameter arrays, kept until it is destroyed
+    for (int k = 0; k < 5; ++k)
+      if ((e = hipMalloc(&a->PV[k], 4 * (size_t)a->cfg.max_voxels))) {
+        for (auto& q : a->PV) {
+          hipFree(q);
+          q = nullptr;
+        }
+        return undo(PHS_ERR_OOM, std::string("per-voxel parameters: ") + hipGetErrorString(e));
+      }
+  }
+  for (int k = 0; hparams && k < 5; ++k)
+    if ((e = hipMemcpyAsync(a->PV[k] + vo, hpv + (size_t)k * N, sz_f, hipMemcpyHostToDevice, s)))
+      return undo(PHS_ERR_HIP, std::string("slot upload: ") + hipGetErrorString(e));
   sr.sched = h.sched_dev;
   *hs = sr;
   if ((e = hipMemcpyAsync(a->bricks + bo, hb, sz_br, hipMemcpyHostToDevice, s)) ||
@@ -774,7 +811,7 @@ int ensure_graph(phs_arena* a, int32_t n_steps, int32_t par0) {
   const int64_t grid = grid_for(a);
   phs_arena::Graph& G = a->g[par0];
   if (G.exec && G.steps == n_steps && G.grid == grid && G.high == a->brick_high &&
-      G.sweeps == a->sweeps_eff && G.flav == a->flav)
+      G.sweeps == a->sweeps_eff && G.flav == a->flav && G.pv == a->PV[0])
     return PHS_OK;
   if (G.exec) hipGraphExecDestroy(G.exec);
   if (G.graph) hipGraphDestroy(G.graph);
@@ -795,6 +832,7 @@ int ensure_graph(phs_arena* a, int32_t n_steps, int32_t par0) {
   G.high = a->brick_high;
   G.sweeps = a->sweeps_eff;
   G.flav = a->flav;
+  G.pv = a->PV[0];
   return PHS_OK;
 }
 
@@ -976,6 +1014,7 @@ int phs_arena_destroy(phs_arena* a) {
   hipFree(a->DACC);
   hipFree(a->VF);
   hipFree(a->diag_acc);
+  for (auto& q : a->PV) hipFree(q);
   delete a;
   return PHS_OK;
 }

@@ -30,18 +30,20 @@ struct PlainParams {
   int model;
   double p0, p1, p2, p3, p4;
   double dt;
+  const double* pv;  // [5][n] per-voxel parameters, or nullptr (uniform p0..p4)
 };
 
-// Reaction right-hand side, operation order of legacy/compute_simulation_cpu.py:144-160.
-__device__ __forceinline__ void rhs64(const PlainParams& P, double u, double h, double J,
-                                      double& du, double& dh) {
+// Reaction right-hand side, operation order of legacy/compute_simulation_cpu.py:144-160;
+// p[k] is the voxel's P_2d[n, 15 + k].
+__device__ __forceinline__ void rhs64(int model, const double (&p)[5], double u, double h,
+                                      double J, double& du, double& dh) {
 #pragma clang fp contract(off)
-  if (P.model == 0) {
-    du = (h * (u * u) * (1.0 - u)) / P.p2 - u / P.p3 + J;
-    dh = (u < P.p4) ? (1.0 - h) / P.p0 : -h / P.p1;
+  if (model == 0) {
+    du = (h * (u * u) * (1.0 - u)) / p[2] - u / p[3] + J;
+    dh = (u < p[4]) ? (1.0 - h) / p[0] : -h / p[1];
   } else {
-    du = -P.p0 * u * (u - P.p1) * (u - 1.0) - u * h + J;
-    dh = (P.p2 + P.p3 * h / (u + P.p4)) * (-h - P.p0 * u * (u - P.p1 - 1.0));
+    du = -p[0] * u * (u - p[1]) * (u - 1.0) - u * h + J;
+    dh = (p[2] + p[3] * h / (u + p[4])) * (-h - p[0] * u * (u - p[1] - 1.0));
   }
 }
 
@@ -60,11 +62,15 @@ __global__ void __launch_bounds__(kBlock) plain_reaction(PlainParams P, double* 
                        : 0.0;
   const double dt = P.dt;
   const double u = U[n], h = H[n];
+  double p[5] = {P.p0, P.p1, P.p2, P.p3, P.p4};
+  if (P.pv)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) p[k] = P.pv[(size_t)k * P.n + n];
   double k1u, k1h, k2u, k2h, k3u, k3h, k4u, k4h;
-  rhs64(P, u, h, J, k1u, k1h);
-  rhs64(P, u + 0.5 * dt * k1u, h + 0.5 * dt * k1h, J, k2u, k2h);
-  rhs64(P, u + 0.5 * dt * k2u, h + 0.5 * dt * k2h, J, k3u, k3h);
-  rhs64(P, u + dt * k3u, h + dt * k3h, J, k4u, k4h);
+  rhs64(P.model, p, u, h, J, k1u, k1h);
+  rhs64(P.model, p, u + 0.5 * dt * k1u, h + 0.5 * dt * k1h, J, k2u, k2h);
+  rhs64(P.model, p, u + 0.5 * dt * k2u, h + 0.5 * dt * k2h, J, k3u, k3h);
+  rhs64(P.model, p, u + dt * k3u, h + dt * k3h, J, k4u, k4h);
   U[n] = u + (dt / 6.0) * (k1u + 2.0 * k2u + 2.0 * k3u + k4u);
   H[n] = h + (dt / 6.0) * (k1h + 2.0 * k2h + 2.0 * k3h + k4h);
 }
@@ -180,7 +186,7 @@ struct phs_plain {
   double lmin = 1.0, lmax = 1.0;
   std::vector<phs::ChebCoef> coef;
   double *U = nullptr, *H = nullptr, *R0 = nullptr, *DA = nullptr, *DB = nullptr,
-         *alpha = nullptr;
+         *alpha = nullptr, *pv = nullptr;
   int* nbd = nullptr;
   uint8_t* vf = nullptr;
   int32_t* probe_ids = nullptr;
@@ -223,7 +229,7 @@ int phs_plain_create(const phs_sim_desc* d, const phs_outputs* out, double tol, 
   const int N = d->n_voxel;
   p->N = N;
   p->P = PlainParams{N, d->model, d->params[0], d->params[1], d->params[2], d->params[3],
-                     d->params[4], d->dt};
+                     d->params[4], d->dt, nullptr};
   p->n_steps = d->n_steps;
   p->steps_per_sample = d->steps_per_sample;
   p->n_samples = d->n_samples;
@@ -269,8 +275,14 @@ int phs_plain_create(const phs_sim_desc* d, const phs_outputs* out, double tol, 
   if ((e = hipMalloc(&p->U, nd)) || (e = hipMalloc(&p->H, nd)) || (e = hipMalloc(&p->R0, nd)) ||
       (e = hipMalloc(&p->DA, nd)) || (e = hipMalloc(&p->DB, nd)) ||
       (e = hipMalloc(&p->alpha, nd)) || (e = hipMalloc(&p->nbd, sizeof(int) * 6 * (size_t)N)) ||
-      (e = hipMalloc(&p->vf, (size_t)N)))
+      (e = hipMalloc(&p->vf, (size_t)N)) ||
+      (d->params_voxel && (e = hipMalloc(&p->pv, 5 * nd))))
     return cleanup(phs::fail(PHS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e)));
+  if (d->params_voxel) {
+    if ((e = hipMemcpy(p->pv, d->params_voxel, 5 * nd, hipMemcpyHostToDevice)))
+      return cleanup(phs::fail(PHS_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
+    p->P.pv = p->pv;
+  }
   std::vector<double> u0(N), h0(N);
   for (int i = 0; i < N; ++i) {
     u0[i] = d->u0 ? (double)d->u0[i] : 0.0;
@@ -387,6 +399,7 @@ int phs_plain_destroy(phs_plain* p) {
   hipFree(p->DA);
   hipFree(p->DB);
   hipFree(p->alpha);
+  hipFree(p->pv);
   hipFree(p->nbd);
   hipFree(p->vf);
   hipFree(p->probe_ids);

@@ -42,6 +42,7 @@ class Prepared:
     lat_threshold: float = LAT_THRESHOLD
     u0: np.ndarray | None = None
     h0: np.ndarray | None = None
+    params_voxel: np.ndarray | None = None  # float64 5 x N when P_2d[:, 15:20] varies
     keep: list = field(default_factory=list)
 
     def physical_time(self) -> np.ndarray:
@@ -81,6 +82,7 @@ class Prepared:
         d.u0 = _lib.ptr(self.u0)
         d.h0 = _lib.ptr(self.h0)
         d.lat_threshold = float(self.lat_threshold)
+        d.params_voxel = _lib.ptr(self.params_voxel)
         return d
 
 
@@ -105,12 +107,16 @@ def prepare(n_voxel, P_2d, geometry_data, simulation_parameters, arrhythmia_para
     nb = np.asarray(geometry_data["neighbor_id_2d"])
     delta_k = (nb[:, :6] >= 0).astype(np.float64)
     if np.any(P[:, 6:15] != 0) or np.any(P[:, 0:6] != 4.0 * delta_k):
-        raise NotImplementedError("anisotropic diffusion (P_2d columns 0-14 other than D = I) "
-                                  "is not supported by the device path")
+        # the C ABI carries no diffusion tensor: the device path is the reference's D = I
+        # (fiber_flag 0, simulation/equation_parts.py:19, :32)
+        raise _lib.PhsError(f"PHS_ERR_UNSUPPORTED ({_lib.PHS_ERR_UNSUPPORTED}): anisotropic "
+                            "diffusion (P_2d columns 0-14 other than D = I) is not supported by "
+                            "the device path")
     params = P[0, 15:20].copy()
+    params_voxel = None
     if np.any(P[:, 15:20] != params[None, :]):
-        raise NotImplementedError("per-voxel model parameters (non-uniform P_2d[:, 15:20]) are "
-                                  "not supported by the device path")
+        # per-voxel cell parameters, read per voxel as simulation_gpu.py:29-41 does
+        params_voxel = np.ascontiguousarray(P[:, 15:20].T, dtype=np.float64)
     sim, arr = simulation_parameters, arrhythmia_parameters
     node_flag = np.asarray(arr["node_flag"]) if "node_flag" in arr else np.zeros(n_voxel, np.int64)
     model = int(sim["heart_model_flag"])
@@ -143,7 +149,7 @@ def prepare(n_voxel, P_2d, geometry_data, simulation_parameters, arrhythmia_para
         delta=float(geometry_data["Delta"]), time_scale=ts, n_steps=n_steps,
         steps_per_sample=nspm, n_samples=n_samples, ijk=np.ascontiguousarray(ijk, np.int32),
         nb=np.ascontiguousarray(nb[:, :6], dtype=np.int32), vflags=vflags,
-        schedule=np.ascontiguousarray(schedule, np.uint8))
+        schedule=np.ascontiguousarray(schedule, np.uint8), params_voxel=params_voxel)
 
 
 def brick_plan(ijk, nb=None, brick=(4, 4, 4)):

@@ -66,16 +66,20 @@ def test_prepare_flags_schedule_sampling():
         np.array_equal(prep.ijk, w["geometry"]["ijk"])
 
 
-def test_prepare_rejects_anisotropy_and_hetero_params():
+def test_prepare_rejects_anisotropy_keeps_hetero_params():
     w = small_shell()
     P = w["P_2d"].copy()
     P[3, 7] = 0.5
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(_lib.PhsError, match="PHS_ERR_UNSUPPORTED"):
         runtime.prepare(P.shape[0], P, w["geometry"], w["sim"], w["arr"])
     P = w["P_2d"].copy()
     P[3, 17] = 0.31
-    with pytest.raises(NotImplementedError):
-        runtime.prepare(P.shape[0], P, w["geometry"], w["sim"], w["arr"])
+    prep = runtime.prepare(P.shape[0], P, w["geometry"], w["sim"], w["arr"])
+    assert prep.params_voxel.shape == (5, P.shape[0])
+    assert np.array_equal(prep.params_voxel, P[:, 15:20].T)
+    assert prep.desc().params_voxel == prep.params_voxel.ctypes.data
+    uniform = runtime.prepare(P.shape[0], w["P_2d"], w["geometry"], w["sim"], w["arr"])
+    assert uniform.params_voxel is None and uniform.desc().params_voxel is None
 
 
 def test_lattice_recovered_from_neighbours():
