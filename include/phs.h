@@ -156,6 +156,11 @@ int phs_plain_create(const phs_sim_desc* desc, const phs_outputs* out, double to
 int phs_plain_run(phs_plain* handle, int64_t n_steps, void* stream);
 /* Device-to-device copy of the current state (double[n_voxel] each; either may be NULL). */
 int phs_plain_state(phs_plain* handle, double* u, double* h, void* stream);
+/* Divergence report: the first step whose new transmembrane potential was not finite (NaN or
+ * inf) at any voxel, or -1.  Recorded on the device by the step itself; reading it waits for
+ * the work queued on `stream`.  The reference has no such report (simulation_gpu.py:250-255
+ * only prints non-convergence of the solver). */
+int phs_plain_status(phs_plain* handle, int64_t* first_nonfinite_step, void* stream);
 /* Chebyshev sweeps per step and the spectral bounds used. */
 int phs_plain_info(phs_plain* handle, int32_t* sweeps, double* lambda_min, double* lambda_max);
 int phs_plain_destroy(phs_plain* handle);
@@ -223,6 +228,10 @@ int phs_arena_perm(phs_arena* a, int32_t slot, int32_t* perm_host);
  * next step's reaction runs at the end of each step, so between steps u is that reaction's
  * u* (and h its h); after a simulation's last step it is the final state. */
 int phs_arena_state(phs_arena* a, int32_t slot, float* u, float* h, void* stream);
+/* Divergence report of one slot, as phs_plain_status: the first step whose new u was not
+ * finite at any of its voxels, or -1. */
+int phs_arena_slot_status(phs_arena* a, int32_t slot, int64_t* first_nonfinite_step,
+                          void* stream);
 /* Chebyshev sweeps and spectral bounds used for a slot. */
 int phs_arena_slot_bounds(phs_arena* a, int32_t slot, double* lambda_min, double* lambda_max);
 /* FP64 relative residual ||b - A x|| / ||b|| of the last CN solve of a slot, recomputed on

@@ -104,6 +104,7 @@ SIGNATURES = {
     "phs_plain_run": (C.c_int, [_P, _I64, _P]),
     "phs_plain_state": (C.c_int, [_P, _P, _P, _P]),
     "phs_plain_info": (C.c_int, [_P, _PI32, _PD, _PD]),
+    "phs_plain_status": (C.c_int, [_P, _PI64, _P]),
     "phs_plain_destroy": (C.c_int, [_P]),
     "phs_arena_create": (C.c_int, [C.POINTER(ArenaConfig), C.POINTER(_P)]),
     "phs_arena_add": (C.c_int, [_P, C.POINTER(SimDesc), C.POINTER(Outputs), _PI32, _P]),
@@ -117,6 +118,7 @@ SIGNATURES = {
     "phs_arena_perm": (C.c_int, [_P, _I32, _P]),
     "phs_arena_state": (C.c_int, [_P, _I32, _P, _P, _P]),
     "phs_arena_slot_bounds": (C.c_int, [_P, _I32, _PD, _PD]),
+    "phs_arena_slot_status": (C.c_int, [_P, _I32, _PI64, _P]),
     "phs_arena_last_residual": (C.c_int, [_P, _I32, _PD, _P]),
     "phs_arena_sweeps": (C.c_int, [_P, _PI32]),
     "phs_arena_profile": (C.c_int, [_P, _I32, _PD, _P]),

@@ -100,6 +100,7 @@ struct ArenaDev {
               // formed in FP32 when D1 is written
   const float* PV[5];  // per-voxel cell parameters (same encoding as SlotRec::p) of the
                        // simulations with hparams, or null
+  int32_t* NONFINITE;  // per slot: first step whose new u is not finite (INT32_MAX = none)
   int32_t n_bricks;
   int32_t flags;     // bit 0: XCD remap; bit 1: launch parity of this step (the slots' copy
                      // of the per-step values).  One word with n_bricks, so that every kernel
@@ -426,11 +427,12 @@ struct SlotFinish {
   float* su;
   float* sh;
   int32_t* lat;
-  int32_t n_voxel, max_beats, next_sched, model, hparams;
+  int32_t n_voxel, max_beats, next_sched, model, hparams, slot;
   float p[5];
   float dt, lat_thr;
-  __device__ __forceinline__ void load(const SlotRec& S, int par) {
+  __device__ __forceinline__ void load(const SlotRec& S, int par, int slot_id) {
     hparams = S.hparams;
+    slot = slot_id;
     step = S.dyn[par].step;
     n_steps = S.n_steps;
     voxel_off = S.voxel_off;
@@ -583,6 +585,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
 __device__ __forceinline__ void finish_voxel(const ArenaDev& A, const SlotFinish& S, int idx,
                                              float x, uint8_t f, float hv,
                                              const float (&p)[5]) {
+  if (!__builtin_isfinite(x)) atomicMin(A.NONFINITE + S.slot, (int32_t)S.step);
   const bool now = x >= S.lat_thr;
   const bool above = (f & PHS_VF_ABOVE) != 0;
   const int64_t local = (int64_t)idx - S.voxel_off;
@@ -658,7 +661,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       if (MODE == MODE_DACC_ADD || MODE == MODE_DACC_ADD_X) pd[j] = A.DACC[i];
       if (MODE == MODE_DACC_SET) pd[j] = GUESS >= 1 ? (double)guess_array<GUESS>(A)[i] : 0.0;
       if (kFinish) {
-        sf[j].load(A.slots[br.slot], par_of(A));
+        sf[j].load(A.slots[br.slot], par_of(A), br.slot);
         pu[j] = ld(A.U, i);
         ph[j] = ld(A.H, i);
         pf[j] = ld(A.VF, i);

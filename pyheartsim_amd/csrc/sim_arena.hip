@@ -112,6 +112,7 @@ struct phs_arena {
   uint8_t* VF = nullptr;
   double* diag_acc = nullptr;
   float* PV[5] = {};  // per-voxel cell parameters, allocated by the first add that has them
+  int32_t* NONFINITE = nullptr;  // [max_sims] first step with a non-finite u per slot
   std::vector<SlotHost> sh;
   RangeAlloc vox, brk;
   int64_t brick_high = 0;
@@ -143,6 +144,8 @@ struct phs_arena {
 
 namespace {
 
+const int32_t kNoStep = INT32_MAX;  // "no non-finite step yet", uploaded at each add
+
 enum { KIND_RHS = 0, KIND_FIRST = 1, KIND_MID = 2, KIND_LAST = 3, KIND_RESIDUAL = 4,
        KIND_REFINE = 5 };
 
@@ -171,6 +174,7 @@ ArenaDev dev_view(const phs_arena* a) {
   A.D1 = a->D1;
   A.D2 = a->D2;
   for (int k = 0; k < 5; ++k) A.PV[k] = a->PV[k];
+  A.NONFINITE = a->NONFINITE;
   A.n_bricks = (int32_t)a->brick_high;
   A.flags = (a->cfg.xcd_remap ? 1 : 0) | (a->par << 1);
   A.diag = a->cfg.diag ? 1 : 0;
@@ -459,6 +463,7 @@ int phs_arena_create(const phs_arena_config* cfg, phs_arena** handle) {
       (e = hipMalloc(&a->R0, 4 * nv)) || (e = hipMalloc(&a->DA, 4 * nv)) ||
       (e = hipMalloc(&a->DB, 4 * nv)) || (e = hipMalloc(&a->VF, nv)) ||
       (e = hipMalloc(&a->ALPHA, 4 * nv)) || (e = hipMalloc(&a->diag_acc, 16)) ||
+      (e = hipMalloc(&a->NONFINITE, 4 * (size_t)cfg->max_sims)) ||
       (need_dacc && (e = hipMalloc(&a->DACC, 8 * nv))) ||
       (cfg->guess >= 1 && (e = hipMalloc(&a->D1, 4 * nv))) ||
       (cfg->guess == 2 && (e = hipMalloc(&a->D2, 4 * nv))) ||
@@ -753,6 +758,7 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
       (a->D1 && (e = hipMemsetAsync(a->D1 + vo, 0, sz_f, s))) ||
       (a->D2 && (e = hipMemsetAsync(a->D2 + vo, 0, sz_f, s))) ||
       (e = hipMemcpyAsync(a->slots + slot, hs, sz_sl, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(a->NONFINITE + slot, &kNoStep, 4, hipMemcpyHostToDevice, s)) ||
       (e = launch_react_range(a, (int)bo, (int)(bo + P.n_bricks), s)) ||
       (e = hipEventCreateWithFlags(&h.upload_done, hipEventDisableTiming)) ||
       (e = hipEventRecord(h.upload_done, s)))
@@ -930,6 +936,18 @@ int phs_arena_state(phs_arena* a, int32_t slot, float* u, float* h, void* stream
   return PHS_OK;
 }
 
+int phs_arena_slot_status(phs_arena* a, int32_t slot, int64_t* first_nonfinite_step,
+                          void* stream) {
+  if (!a || !first_nonfinite_step || slot < 0 || slot >= a->cfg.max_sims || !a->sh[slot].used)
+    return phs::fail(PHS_ERR_ARG, "invalid slot");
+  int32_t v = INT32_MAX;
+  hipStream_t s = (hipStream_t)stream;
+  PHS_HIP_CHECK(hipMemcpyAsync(&v, a->NONFINITE + slot, sizeof v, hipMemcpyDeviceToHost, s));
+  PHS_HIP_CHECK(hipStreamSynchronize(s));
+  *first_nonfinite_step = v == INT32_MAX ? -1 : (int64_t)v;
+  return PHS_OK;
+}
+
 int phs_arena_slot_bounds(phs_arena* a, int32_t slot, double* lmin, double* lmax) {
   if (!a || slot < 0 || slot >= a->cfg.max_sims || !a->sh[slot].used)
     return phs::fail(PHS_ERR_ARG, "invalid slot");
@@ -1014,6 +1032,7 @@ int phs_arena_destroy(phs_arena* a) {
   hipFree(a->DACC);
   hipFree(a->VF);
   hipFree(a->diag_acc);
+  hipFree(a->NONFINITE);
   for (auto& q : a->PV) hipFree(q);
   delete a;
   return PHS_OK;

@@ -112,6 +112,7 @@ struct SweepOut {
   void* sh;
   int fp64;
   long row;
+  int32_t* nonfinite;  // first step whose new u is not finite (INT32_MAX = none)
 };
 
 // One Chebyshev sweep d_{k+1} = a d_k + b d_{k-1} + g (r0 - A d_k).
@@ -147,6 +148,7 @@ __global__ void __launch_bounds__(kBlock) plain_sweep(int N, Nb6 nb, const doubl
   }
   const double x = fo.U[n] + dnew;
   fo.U[n] = x;
+  if (!__builtin_isfinite(x)) atomicMin(fo.nonfinite, (int32_t)fo.step);
   uint8_t f = fo.vf[n];
   const bool now = x >= fo.thr;
   const bool above = (f & PHS_VF_ABOVE) != 0;
@@ -190,6 +192,7 @@ struct phs_plain {
   int* nbd = nullptr;
   uint8_t* vf = nullptr;
   int32_t* probe_ids = nullptr;
+  int32_t* nonfinite = nullptr;  // device: first step with a non-finite u (INT32_MAX = none)
   std::vector<uint8_t> sched;
   phs_outputs out{};
   int64_t n_steps = 0;
@@ -276,8 +279,12 @@ int phs_plain_create(const phs_sim_desc* d, const phs_outputs* out, double tol, 
       (e = hipMalloc(&p->DA, nd)) || (e = hipMalloc(&p->DB, nd)) ||
       (e = hipMalloc(&p->alpha, nd)) || (e = hipMalloc(&p->nbd, sizeof(int) * 6 * (size_t)N)) ||
       (e = hipMalloc(&p->vf, (size_t)N)) ||
-      (d->params_voxel && (e = hipMalloc(&p->pv, 5 * nd))))
+      (d->params_voxel && (e = hipMalloc(&p->pv, 5 * nd))) ||
+      (e = hipMalloc(&p->nonfinite, sizeof(int32_t))))
     return cleanup(phs::fail(PHS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e)));
+  const int32_t none = INT32_MAX;
+  if ((e = hipMemcpy(p->nonfinite, &none, sizeof none, hipMemcpyHostToDevice)))
+    return cleanup(phs::fail(PHS_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
   if (d->params_voxel) {
     if ((e = hipMemcpy(p->pv, d->params_voxel, 5 * nd, hipMemcpyHostToDevice)))
       return cleanup(phs::fail(PHS_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)));
@@ -338,6 +345,7 @@ int phs_plain_run(phs_plain* p, int64_t n_steps, void* stream) {
     fo.su = p->out.samples_u;
     fo.sh = p->out.samples_h;
     fo.fp64 = p->out.sample_fp64;
+    fo.nonfinite = p->nonfinite;
     const bool save = p->steps_per_sample > 0 ? (p->step % p->steps_per_sample == 0) : true;
     fo.row = (save && p->id_save < p->n_samples) ? (long)p->id_save : -1;
     if (fo.row >= 0) p->id_save++;
@@ -391,6 +399,16 @@ int phs_plain_info(phs_plain* p, int32_t* sweeps, double* lmin, double* lmax) {
   return PHS_OK;
 }
 
+int phs_plain_status(phs_plain* p, int64_t* first_nonfinite_step, void* stream) {
+  if (!p || !first_nonfinite_step) return phs::fail(PHS_ERR_ARG, "NULL argument");
+  int32_t v = INT32_MAX;
+  hipStream_t s = (hipStream_t)stream;
+  PHS_HIP_CHECK(hipMemcpyAsync(&v, p->nonfinite, sizeof v, hipMemcpyDeviceToHost, s));
+  PHS_HIP_CHECK(hipStreamSynchronize(s));
+  *first_nonfinite_step = v == INT32_MAX ? -1 : (int64_t)v;
+  return PHS_OK;
+}
+
 int phs_plain_destroy(phs_plain* p) {
   if (!p) return PHS_OK;
   hipFree(p->U);
@@ -400,6 +418,7 @@ int phs_plain_destroy(phs_plain* p) {
   hipFree(p->DB);
   hipFree(p->alpha);
   hipFree(p->pv);
+  hipFree(p->nonfinite);
   hipFree(p->nbd);
   hipFree(p->vf);
   hipFree(p->probe_ids);

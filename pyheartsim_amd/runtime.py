@@ -213,6 +213,13 @@ class PlainSim:
         _lib.check(self.lib.phs_plain_run(self.handle, n, _lib.stream_ptr(stream)), "phs_plain_run")
         self.steps_done = min(self.prep.n_steps, self.steps_done + n)
 
+    def nonfinite_step(self) -> int:
+        """First step whose new u was NaN or inf at any voxel, or -1 (phs_plain_status)."""
+        v = C.c_int64()
+        _lib.check(self.lib.phs_plain_status(self.handle, C.byref(v), _lib.stream_ptr()),
+                   "phs_plain_status")
+        return v.value
+
     def state(self):
         torch = self.torch
         u = torch.empty(self.prep.n_voxel, dtype=torch.float64, device="cuda")
@@ -389,6 +396,14 @@ class Arena:
         lo, hi = C.c_double(), C.c_double()
         _lib.check(self.lib.phs_arena_slot_bounds(self.handle, slot, C.byref(lo), C.byref(hi)))
         return lo.value, hi.value
+
+    def nonfinite_step(self, slot: int) -> int:
+        """First step whose new u was NaN or inf at any voxel of the slot, or -1
+        (phs_arena_slot_status)."""
+        v = C.c_int64()
+        _lib.check(self.lib.phs_arena_slot_status(self.handle, slot, C.byref(v), _lib.stream_ptr()),
+                   "phs_arena_slot_status")
+        return v.value
 
     def last_residual(self, slot: int) -> float:
         r = C.c_double()
