@@ -135,9 +135,11 @@ int phs_auto_sweeps(double lambda_max, int32_t guess, int32_t refine, int32_t re
 /* Host only (no device work): the Morton brick plan of a voxel set for brick shape
  * brick[3] (bx*by*bz = 64).  perm [n_voxel] receives the compact -> caller order,
  * *n_bricks the brick count, masks (NULL or capacity n_voxel) the per-brick occupancy.
- * When nb is not NULL the face-neighbour table is verified against the lattice
- * (PHS_ERR_UNSUPPORTED on mismatch).  Replaces nothing in the reference: it is the
- * voxel reordering that the device forms add. */
+ * When nb is not NULL the face-neighbour table is checked against the lattice: each entry
+ * must be the adjacent lattice voxel or -1 (a -1 where that voxel exists cuts the link from
+ * that row of the operator, as the reference's table-built operator does); any other entry
+ * gives PHS_ERR_UNSUPPORTED (the plain FP64 form takes any table).  Replaces nothing in the
+ * reference: it is the voxel reordering that the device forms add. */
 int phs_plan_build(int32_t n_voxel, const int32_t* ijk, const int32_t* nb, int32_t nb_stride,
                    const int32_t* brick, int32_t* perm, int32_t* n_bricks, uint64_t* masks);
 

@@ -70,9 +70,18 @@ __device__ __forceinline__ double w_axis(const double* __restrict__ vox, const d
   return c[v] / (l * l * l) * proj;
 }
 
+// The lead-field column of voxel w is the transpose of the central-difference stencil:
+//   Wt[e, w] = 1/(2 Delta) sum over the stencil terms that read u(w):
+//     + W_k[e, v] for every v whose axis-k pair is complete and p_k(v) = w,
+//     - W_k[e, v] for every v whose axis-k pair is complete and m_k(v) = w.
+// The terms of w are listed on the host (inv_ptr / inv_v / inv_code = 2 k + (minus)), in the
+// order axis k, then + before -, then v, so the sum is formed in a fixed order; for a
+// symmetric table they are + W_k[m_k(w)] and - W_k[p_k(w)].
 __global__ void build_leadfield(int N, int E, int Kpad, const double* __restrict__ vox,
                                 const double* __restrict__ el, const double* __restrict__ c,
-                                const double* __restrict__ D, const int* __restrict__ nb,
+                                const double* __restrict__ D, const int* __restrict__ inv_ptr,
+                                const int* __restrict__ inv_v,
+                                const unsigned char* __restrict__ inv_code,
                                 const int* __restrict__ perm, double inv2d, int Epad,
                                 float* __restrict__ Wf, unsigned short* __restrict__ Wt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -82,13 +91,10 @@ __global__ void build_leadfield(int N, int E, int Kpad, const double* __restrict
   double acc = 0.0;
   if (i < N && e < E) {
     const int w = perm ? perm[i] : i;
-    for (int k = 0; k < 3; ++k) {
-      const int m = nb[6 * (size_t)w + 2 * k + 1];
-      if (m >= 0 && nb[6 * (size_t)m + 2 * k] >= 0 && nb[6 * (size_t)m + 2 * k + 1] >= 0)
-        acc += w_axis(vox, el, c, D, N, m, e, k);
-      const int p = nb[6 * (size_t)w + 2 * k];
-      if (p >= 0 && nb[6 * (size_t)p + 2 * k] >= 0 && nb[6 * (size_t)p + 2 * k + 1] >= 0)
-        acc -= w_axis(vox, el, c, D, N, p, e, k);
+    for (int q = inv_ptr[w]; q < inv_ptr[w + 1]; ++q) {
+      const int code = inv_code[q];
+      const double t = w_axis(vox, el, c, D, N, inv_v[q], e, code >> 1);
+      acc = (code & 1) ? acc - t : acc + t;
     }
     acc *= inv2d;
   }
@@ -610,20 +616,52 @@ int phs_egm_create(int32_t N, int32_t E, const double* voxel, const double* elec
   if (N <= 0 || E <= 0 || !voxel || !electrodes || !c_voxel || !nb || nb_stride < 6 || !handle ||
       !(delta > 0))
     return phs::fail(PHS_ERR_ARG, "invalid electrogram arguments");
-  // The stencil transpose needs a symmetric face table: p_k(m_k(w)) == w.
-  std::vector<int> nb6((size_t)6 * N);
+  // Transpose of the central-difference stencil (simulation/unipolar_electrogram_equation_parts
+  // .py:60-103: the axis-k gradient at v is (u(p_k(v)) - u(m_k(v))) / (2 Delta) when both
+  // exist, else 0), for any face table: the terms reading u(w), grouped by w.
   for (int32_t w = 0; w < N; ++w)
     for (int k = 0; k < 6; ++k) {
       const int32_t m = nb[(int64_t)w * nb_stride + k];
       if (m < -1 || m >= N) return phs::fail(PHS_ERR_ARG, "neighbour id out of range");
-      nb6[(size_t)6 * w + k] = m;
     }
-  for (int32_t w = 0; w < N; ++w)
-    for (int k = 0; k < 6; ++k) {
-      const int32_t m = nb6[(size_t)6 * w + k];
-      if (m >= 0 && nb6[(size_t)6 * m + (k ^ 1)] != w)
-        return phs::fail(PHS_ERR_UNSUPPORTED, "neighbour table is not symmetric");
+  std::vector<int32_t> inv_ptr((size_t)N + 1, 0);
+  for (int32_t v = 0; v < N; ++v)
+    for (int k = 0; k < 3; ++k) {
+      const int32_t p = nb[(int64_t)v * nb_stride + 2 * k], m = nb[(int64_t)v * nb_stride + 2 * k + 1];
+      if (p >= 0 && m >= 0) {
+        ++inv_ptr[(size_t)p + 1];
+        ++inv_ptr[(size_t)m + 1];
+      }
     }
+  for (int32_t w = 0; w < N; ++w) inv_ptr[(size_t)w + 1] += inv_ptr[w];
+  std::vector<int32_t> inv_v(inv_ptr[N]);
+  std::vector<unsigned char> inv_code(inv_ptr[N]);
+  {
+    std::vector<int32_t> fill(inv_ptr.begin(), inv_ptr.end() - 1);
+    for (int32_t v = 0; v < N; ++v)
+      for (int k = 0; k < 3; ++k) {
+        const int32_t p = nb[(int64_t)v * nb_stride + 2 * k], m = nb[(int64_t)v * nb_stride + 2 * k + 1];
+        if (p >= 0 && m >= 0) {
+          inv_v[fill[p]] = v;
+          inv_code[fill[p]++] = (unsigned char)(2 * k);
+          inv_v[fill[m]] = v;
+          inv_code[fill[m]++] = (unsigned char)(2 * k + 1);
+        }
+      }
+    // order within each voxel: axis, then + before -, then v (v ascending already)
+    std::vector<std::pair<unsigned char, int32_t>> tmp;
+    for (int32_t w = 0; w < N; ++w) {
+      const int32_t a = inv_ptr[w], b = inv_ptr[(size_t)w + 1];
+      if (b - a <= 1) continue;
+      tmp.clear();
+      for (int32_t q = a; q < b; ++q) tmp.emplace_back(inv_code[q], inv_v[q]);
+      std::sort(tmp.begin(), tmp.end());
+      for (int32_t q = a; q < b; ++q) {
+        inv_code[q] = tmp[q - a].first;
+        inv_v[q] = tmp[q - a].second;
+      }
+    }
+  }
   if (perm) {
     std::vector<char> seen(N, 0);
     for (int32_t i = 0; i < N; ++i) {
@@ -643,14 +681,18 @@ int phs_egm_create(int32_t N, int32_t E, const double* voxel, const double* elec
   g->Epad = (E + 64 * g->ce - 1) / (64 * g->ce) * (64 * g->ce);
   const size_t nw = (size_t)g->Epad * g->Kpad;
   double *dv = nullptr, *de = nullptr, *dc = nullptr, *dD = nullptr;
-  int *dnb = nullptr, *dperm = nullptr;
+  int *dptr = nullptr, *dinv = nullptr, *dperm = nullptr;
+  unsigned char* dcode = nullptr;
+  const size_t n_inv = std::max<size_t>(inv_v.size(), 1);
   hipError_t e = hipSuccess;
   auto cleanup = [&](int code, const std::string& msg) {
     hipFree(dv);
     hipFree(de);
     hipFree(dc);
     hipFree(dD);
-    hipFree(dnb);
+    hipFree(dptr);
+    hipFree(dinv);
+    hipFree(dcode);
     hipFree(dperm);
     if (code) phs_egm_destroy(g);
     return code ? phs::fail(code, msg) : PHS_OK;
@@ -658,18 +700,23 @@ int phs_egm_create(int32_t N, int32_t E, const double* voxel, const double* elec
   if ((e = hipMalloc(&g->Wf, 4 * nw)) || (e = hipMalloc(&g->Wt, 4 * nw)) ||
       (e = hipMalloc(&dv, 24 * (size_t)N)) ||
       (e = hipMalloc(&de, 24 * (size_t)E)) || (e = hipMalloc(&dc, 8 * (size_t)N)) ||
-      (D && (e = hipMalloc(&dD, 72 * (size_t)N))) || (e = hipMalloc(&dnb, 24 * (size_t)N)) ||
+      (D && (e = hipMalloc(&dD, 72 * (size_t)N))) ||
+      (e = hipMalloc(&dptr, 4 * ((size_t)N + 1))) || (e = hipMalloc(&dinv, 4 * n_inv)) ||
+      (e = hipMalloc(&dcode, n_inv)) ||
       (perm && (e = hipMalloc(&dperm, 4 * (size_t)N))))
     return cleanup(PHS_ERR_OOM, std::string("egm hipMalloc: ") + hipGetErrorString(e));
   if ((e = hipMemcpyAsync(dv, voxel, 24 * (size_t)N, hipMemcpyHostToDevice, s)) ||
       (e = hipMemcpyAsync(de, electrodes, 24 * (size_t)E, hipMemcpyHostToDevice, s)) ||
       (e = hipMemcpyAsync(dc, c_voxel, 8 * (size_t)N, hipMemcpyHostToDevice, s)) ||
       (D && (e = hipMemcpyAsync(dD, D, 72 * (size_t)N, hipMemcpyHostToDevice, s))) ||
-      (e = hipMemcpyAsync(dnb, nb6.data(), 24 * (size_t)N, hipMemcpyHostToDevice, s)) ||
+      (e = hipMemcpyAsync(dptr, inv_ptr.data(), 4 * ((size_t)N + 1), hipMemcpyHostToDevice, s)) ||
+      (!inv_v.empty() &&
+       ((e = hipMemcpyAsync(dinv, inv_v.data(), 4 * inv_v.size(), hipMemcpyHostToDevice, s)) ||
+        (e = hipMemcpyAsync(dcode, inv_code.data(), inv_code.size(), hipMemcpyHostToDevice, s)))) ||
       (perm && (e = hipMemcpyAsync(dperm, perm, 4 * (size_t)N, hipMemcpyHostToDevice, s))))
     return cleanup(PHS_ERR_HIP, std::string("egm upload: ") + hipGetErrorString(e));
   const dim3 grid((g->Kpad + 255) / 256, g->Epad);
-  build_leadfield<<<grid, 256, 0, s>>>(N, E, g->Kpad, dv, de, dc, dD, dnb, dperm,
+  build_leadfield<<<grid, 256, 0, s>>>(N, E, g->Kpad, dv, de, dc, dD, dptr, dinv, dcode, dperm,
                                        1.0 / (2.0 * delta), g->Epad, g->Wf, g->Wt);
   if ((e = hipGetLastError()) || (e = hipStreamSynchronize(s)))
     return cleanup(PHS_ERR_HIP, std::string("build_leadfield: ") + hipGetErrorString(e));

@@ -10,9 +10,11 @@
 // Each brick records its 6 face-neighbour bricks (-1 = none).
 //
 // The plan is checked against the caller's neighbour table: face neighbour k of every
-// voxel must be exactly the voxel at the adjacent lattice point (or -1 when that point is
-// empty).  The reference builds the table from coordinates by KD-tree lookup at unit
-// offsets (utility/voxelization.py:102-220).
+// voxel must be the voxel at the adjacent lattice point, or -1.  A -1 where the lattice point
+// is occupied is a cut link, recorded per brick and lane so that that row of the operator
+// leaves the neighbour out, as the reference's operator built from the table does (its rows
+// may then differ from their transposes).  The reference builds the table from coordinates
+// by KD-tree lookup at unit offsets (utility/voxelization.py:102-220).
 #include <algorithm>
 #include <cstdint>
 #include <string>
@@ -112,8 +114,9 @@ int build_plan(int32_t n, const int32_t* ijk, const int32_t* nb, int32_t nb_stri
       auto it = bmap.find(morton3((uint32_t)x, (uint32_t)y, (uint32_t)z));
       if (it != bmap.end()) P.nbb[(size_t)b * 6 + d] = it->second;
     }
+  P.cut.assign((size_t)nbk * 6, 0);
   if (!nb) return 0;
-  // Verify the caller's face-neighbour table against the lattice.
+  // Verify the caller's face-neighbour table against the lattice; record its cut links.
   for (int32_t b = 0; b < nbk; ++b) {
     const uint64_t m = P.mask[b];
     for (int lane = 0; lane < 64; ++lane) {
@@ -138,8 +141,10 @@ int build_plan(int32_t n, const int32_t* ijk, const int32_t* nb, int32_t nb_stri
             expect = P.perm[P.start[tb] + __builtin_popcountll(tm & ((1ULL << cell) - 1))];
         }
         const int32_t given = nb[(int64_t)o * nb_stride + d];
-        if (given != expect) {
-          err = "neighbour table differs from lattice face adjacency at voxel " +
+        if (given == -1 && expect >= 0) {
+          P.cut[(size_t)b * 6 + d] |= 1ULL << lane;
+        } else if (given != expect) {
+          err = "neighbour table entry is not the adjacent lattice voxel at voxel " +
                 std::to_string(o) + " direction " + std::to_string(d);
           return -2;
         }

@@ -590,9 +590,11 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
       const uint64_t nmask = nb >= 0 ? P.mask[nb] : 0;
       // + direction: the neighbour of lane l is lane l + st (in the brick) or the neighbour
       // brick's cell l - wrap; - direction: l - st, or the cell l + wrap
-      r.pin[k] = ((k & 1) ? shl(mask, st) : shr(mask, st)) & inb[k] & mask;
+      // presence from the lattice, less the links the caller's table cuts
+      const uint64_t keep = ~P.cut[(size_t)b * 6 + k];
+      r.pin[k] = ((k & 1) ? shl(mask, st) : shr(mask, st)) & inb[k] & mask & keep;
       r.sh[k] = (k & 1) ? shr(nmask, wrap) : shl(nmask, wrap);
-      r.pcr[k] = r.sh[k] & ~inb[k] & mask;
+      r.pcr[k] = r.sh[k] & ~inb[k] & mask & keep;
       int32_t below = 0;
       if (nb >= 0 && (k & 1)) {
         // - face: the gather rank is counted from the face-adjacent offset (B-1)*stride

@@ -94,3 +94,21 @@ def test_ap_time_scaling_like_configuration():
     sim, arr, hm = configs.assign_simulation_parameters(g, 0, heart_model_flag=1)
     assert sim["time_scale"] == 6 and sim["dt"] == 0.5 / 6 and sim["t_final"] == 1000 / 6
     assert arr["pacing_start_time"] == 10 / 6 and np.all(hm["c_voxel"] == 1.6 * 6)
+
+
+def test_brick_plan_accepts_cut_links_rejects_foreign_neighbours():
+    """A table may omit lattice neighbours (asymmetric cuts included); an entry that is not
+    the adjacent lattice voxel is refused as unsupported."""
+    g = geo.shell((5.0, 4.0, 4.0), 1.0, 0.5, n_holes=1, hole_radius=1.5, seed=2)
+    nb = np.ascontiguousarray(g["neighbor_id_2d"][:, :6], dtype=np.int32)
+    cut = nb.copy()
+    rng = np.random.default_rng(0)
+    rows = rng.choice(len(nb), 40, replace=False)
+    cut[rows, 0] = -1                       # +x links cut one way only
+    perm, masks = runtime.brick_plan(g["ijk"], cut)
+    assert sorted(perm.tolist()) == list(range(len(nb)))
+    bad = nb.copy()
+    v = int(np.nonzero(nb[:, 0] >= 0)[0][0])
+    bad[v, 0] = (nb[v, 0] + 7) % len(nb)    # some other voxel
+    with pytest.raises(_lib.PhsError, match="not the adjacent lattice voxel"):
+        runtime.brick_plan(g["ijk"], bad)
