@@ -45,7 +45,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
-BRICK_RECORD_BYTES = 192  # sizeof(BrickRec): masks, gather bases and slot of one 64-cell brick
+# sizeof(BrickRec): masks, gather bases, slot and temporary-block lanes of one 64-cell brick
+BRICK_RECORD_BYTES = 192
 METRIC = "voxel-steps/s at LAT err<=1dt (chosen vs plain FP64); p50/p99 sim latency"
 
 PROFILES = os.path.join(ROOT, "profiles")
@@ -130,16 +131,15 @@ def lat_error(a, b, n_steps=None):
 
 def step_bytes(cnt, sweeps, guess=2, samples_per_step=0.0):
     """Algorithmic HBM bytes of one arena step: each array element read or written once by
-    each kernel, the 192-byte brick record once per kernel and brick, the flag byte of
-    scar / temporary-block simulations in every kernel that reads it, and the sample row.
-    Kernels: right-hand side (u*, warm start in; r out), first sweep (r in; d out), sweeps
-    2..m-1 (d_k, d_k-1, r in; d_k+1 out), last sweep (d_k, d_k-1, r, u*, h, flags, d_1, warm
-    start in; u*, h, d_1, warm start out, next step's reaction included)."""
-    V, B, F = cnt["voxels"], cnt["bricks"], cnt["vf_voxels"]
+    each kernel, the brick record once per kernel and brick (it carries the scar and
+    temporary-block lanes, so no other kernel than the last reads a flag byte), and the
+    sample row.  Kernels: right-hand side (u*, warm start in; r out), first sweep (r in; d
+    out), sweeps 2..m-1 (d_k, d_k-1, r in; d_k+1 out), last sweep (d_k, d_k-1, r, u*, h,
+    flags, d_1, warm start in; u*, h, d_1, warm start out, next step's reaction included)."""
+    V, B = cnt["voxels"], cnt["bricks"]
     rec = B * BRICK_RECORD_BYTES
     last = 50 if guess == 2 else (42 if guess == 1 else 34)
-    per = dict(rhs=12 * V + rec + F, first=8 * V + rec + F, mid=16 * V + rec + F,
-               last=last * V + rec)
+    per = dict(rhs=12 * V + rec, first=8 * V + rec, mid=16 * V + rec, last=last * V + rec)
     total = per["rhs"] + per["first"] + (sweeps - 3) * per["mid"] + per["last"] + \
         4 * V * samples_per_step
     return total, per

