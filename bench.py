@@ -620,19 +620,26 @@ def run_shell(args, torch, runtime, pre):
 
 
 def run_egm(args, torch, runtime, pre):
-    """The electrogram alone on the 0.4 mm shell: E electrodes x 2000 samples over the 199 446
-    voxels (BF16 hi/lo MFMA contraction with the precomputed lead field), synthetic
+    """The electrogram alone: E electrodes x T samples over the 0.4 mm shell's 199 446 voxels or
+    the 0.2 mm shell's 1.6 M (BF16 hi/lo MFMA contraction with the precomputed lead field), synthetic
     plane-wave samples, --steps timed launches after --warmup; rel-L2 against the FP32 MFMA
     form on the same samples.  One launch reads the samples (4 T N bytes) and the BF16 hi/lo
     lead field (4 E N) and writes the T x E result."""
     from pyheartsim_amd import geometry as geo
     from pyheartsim_amd.simulation.unipolar_electrogram_gpu import LeadField
-    m = pre["members"][0]
-    N = m["prep"].n_voxel
-    E, T = args.electrodes, 2000
+    E, T = args.electrodes, args.samples
     el = geo.fibonacci_electrodes(E, 15.0)
-    lf = LeadField(el, m["voxel"], None, m["c_voxel"], m["delta"], m["prep"].nb)
-    x = torch.from_numpy(m["voxel"][:, 0].astype(np.float32)).cuda()
+    if args.egm_geometry == "fine":
+        g = pre["fine"]["geometry"]
+        voxel, name = g["voxel"], "0.2 mm precision shell"
+        lf = LeadField(el, voxel, None, pre["fine"]["hm"]["c_voxel"], g["Delta"],
+                       g["neighbor_id_2d"])
+    else:
+        m = pre["members"][0]
+        voxel, name = m["voxel"], "0.4 mm shell"
+        lf = LeadField(el, voxel, None, m["c_voxel"], m["delta"], m["prep"].nb)
+    N = voxel.shape[0]
+    x = torch.from_numpy(voxel[:, 0].astype(np.float32)).cuda()
     tt = torch.arange(T, device="cuda", dtype=torch.float32)[:, None]
     samples = torch.sigmoid(tt * 0.03 - (x[None, :] + 25.0) * 0.3)   # a plane wave in x
     e_ref = lf.run(samples, precision="fp32", normalize=False)
@@ -672,7 +679,7 @@ def run_egm(args, torch, runtime, pre):
         "n_gpus": 1, "steps": K, "warmup": max(args.warmup, 1), "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
         "data": "synthetic",
-        "config": {"workload": f"egm: 0.4 mm shell lead field, E={E} electrodes, T={T} samples, "
+        "config": {"workload": f"egm: {name} lead field, E={E} electrodes, T={T} samples, "
                                f"N={N} voxels", "electrodes": E, "samples": T, "n_voxel": N},
         "roofline": roof, "rel_l2_vs_fp32_mfma": rel, "cpu_baseline": None,
     }
@@ -777,6 +784,10 @@ def parser():
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--qps", type=float, default=4.0)
     ap.add_argument("--electrodes", type=int, default=64, help="egm workload")
+    ap.add_argument("--samples", type=int, default=2000, help="egm workload: time samples")
+    ap.add_argument("--egm-geometry", choices=["shell", "fine"], default="shell",
+                    help="egm workload: the 0.4 mm shell (199 446 voxels) or the 0.2 mm "
+                         "precision shell (1.6 M voxels)")
     ap.add_argument("--precision-ms", type=float, default=300.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-from-step", type=int, default=3000)
@@ -814,6 +825,8 @@ def run(argv=None) -> dict:
     if args.workload == "batched":
         pre["kws"] = configs.ragged_params(args.sims)
         pre["members"] = build_parallel(_batched_member, pre["kws"], args.workers)
+    elif args.workload == "egm" and args.egm_geometry == "fine":
+        pre["fine"] = configs.shell_fine(t_final=1.0)
     elif args.workload in ("shell", "egm"):
         pre["members"] = build_parallel(_shell_member, [0], 1)
     if args.workload in ("batched", "serve") and args.serve_requests > 0:

@@ -87,7 +87,12 @@ struct SlotRec {
 };
 
 struct ArenaDev {
+  // The brick table, its length and the order bits first: one 16-byte kernel-argument load
+  // gives every kernel what it needs to address its brick record.
   const BrickRec* __restrict__ bricks;
+  int32_t n_bricks;
+  int32_t flags;  // bit 0: XCD remap; bit 1: launch parity of this step (the slots' copy of
+                  // the per-step values); bit 2: walk the bricks backwards
   SlotRec* slots;
   float* U;
   float* H;
@@ -104,15 +109,12 @@ struct ArenaDev {
   const float* PV[5];  // per-voxel cell parameters (same encoding as SlotRec::p) of the
                        // simulations with hparams, or null
   int32_t* NONFINITE;  // per slot: first step whose new u is not finite (INT32_MAX = none)
-  int32_t n_bricks;
-  int32_t flags;     // bit 0: XCD remap; bit 1: launch parity of this step (the slots' copy
-                     // of the per-step values).  One word with n_bricks, so that every kernel
-                     // has both from its first kernel-argument load.
   int32_t diag;
   int32_t max_sims;  // slots to advance
 };
 
-__device__ __forceinline__ int xcd_of(const ArenaDev& A) { return A.flags & 1; }
+// The brick-order bits of flags (XCD remap, backwards walk) for first_brick.
+__device__ __forceinline__ int xcd_of(const ArenaDev& A) { return A.flags & 5; }
 __device__ __forceinline__ int par_of(const ArenaDev& A) { return (A.flags >> 1) & 1; }
 
 enum { MODE_MID = 0, MODE_FINAL_X = 1, MODE_DACC_SET = 2, MODE_DACC_ADD = 3, MODE_DACC_ADD_X = 4 };
@@ -153,17 +155,21 @@ __device__ __forceinline__ int mbcnt64(uint64_t m) {
                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// First brick of this wavefront.  With `xcd`, the round-robin dispatch of workgroups over
-// the 8 XCDs is undone so that each XCD works on a contiguous run of bricks and the halo
-// gathers hit that XCD's L2 (the grid is then a multiple of 8).  The remap is computed
-// either way and selected, so that the kernel-argument loads issue together, not behind a
-// branch.
+// First brick of this wavefront.  With bit 0 of `order`, the round-robin dispatch of
+// workgroups over the 8 XCDs is undone so that each XCD works on a contiguous run of bricks
+// and the halo gathers hit that XCD's L2 (the grid is then a multiple of 8).  With bit 2 the
+// bricks are walked backwards (each XCD its own run), so that a launch after a forward one
+// starts on the bricks whose data that launch touched last, still in L2 and the memory-side
+// cache.  The mappings are computed either way and selected, so that the kernel-argument
+// loads issue together, not behind a branch.
 template <int NB>
-__device__ __forceinline__ int first_brick(int xcd) {
+__device__ __forceinline__ int first_brick(int order) {
   const int blk = blockIdx.x;
-  const int per = gridDim.x >> 3;
-  const int remapped = (blk & 7) * per + (blk >> 3);
-  const int b = xcd ? remapped : blk;
+  const int g = gridDim.x;
+  const int per = g >> 3;
+  const int fwd = (order & 1) ? (blk & 7) * per + (blk >> 3) : blk;
+  const int bwd = (order & 1) ? (blk & 7) * per + (per - 1 - (blk >> 3)) : g - 1 - blk;
+  const int b = (order & 4) ? bwd : fwd;
   return __builtin_amdgcn_readfirstlane((b * kWaves + (int)(threadIdx.x >> 6)) * NB);
 }
 

@@ -186,82 +186,57 @@ __device__ __forceinline__ void split_pair(float x0, float x1, unsigned& hi, uns
   lo = __builtin_bit_cast(unsigned, l);
 }
 
-constexpr int kPitch = kK2 + 8;      // LDS row pitch in bf16 (16 B pad against bank conflicts)
 constexpr int kGemmThreads = 512;    // 8 waves: 2 across time x 4 across electrodes
-// 16-row time blocks per wave (workgroup: 32 RW time rows): 4 for up to 128 electrodes (the
-// sample stream dominates), 8 for wider electrode sets (halves the lead-field reads per flop)
+// 16-row time blocks per wave (workgroup: 32 RW time rows) and K depth of one LDS stage:
+// RW 4, 64-deep stages for up to 128 electrodes (the sample stream dominates); for wider
+// electrode sets RW 8 (halves the lead-field reads per flop) with 32-deep stages, which keep
+// the 128 accumulator registers, the staged stage and the operands within the 256 registers
+// a wave has at 2 waves per SIMD (no spills).
 constexpr int rw_for(int ce) { return ce == 4 ? 8 : 4; }
+constexpr int ks_for(int ce) { return ce == 4 ? 32 : 64; }
 
-// egm partial (T x Epad, one K split), samples and lead field staged through LDS.
-// Workgroup = 8 waves: wave (tg, w) = (wave / 4, wave % 4) owns time rows
-// [tb + 16 RW tg, tb + 16 RW (tg + 1)) and electrodes [e0 + 16 CE w, e0 + 16 CE (w + 1)) as
-// RW x CE tiles of 16 x 16 with FP32 accumulators (workgroup: 32 RW time rows).  Per 64-deep K
-// stage the workgroup
-//   * reads its (32 RW) x 64 FP32 sample tile from HBM once (256 B per time row) and splits it
-//     into BF16 hi/lo planes in LDS;
-//   * copies its (64 CE) x 64 lead-field tile (BF16 hi/lo planes, L2/MALL-resident across
-//     time tiles) into LDS, once for both time groups;
-//   * runs every wave's MFMAs with both operands from LDS.
-// The next stage's tiles are loaded into registers while the current stage's MFMAs run.
-// LO: add the hi*lo and lo*hi products.
-// Operand layout of v_mfma_f32_16x16x32_bf16: lane (r16 = lane % 16, q = lane / 16) holds
-// A row r16 and B column r16, k = 8q .. 8q+7; the result lane holds column r16, rows 4q .. 4q+3.
-//
-// Tile order (1D grid, mt time tiles x ncols (electrode tile, K split) columns, the column count
-// padded to a multiple of 8): workgroups are dispatched to the 8 XCDs round-robin, so workgroup
-// id runs on XCD id % 8.  Slot id / 8 of XCD x walks the mt time tiles of column
-// 8 (slot / mt) + x, i.e. all time tiles of a column run on one XCD at about the same time
-// and read each lead-field stage once from HBM, then from that XCD's L2.
-template <int RW, int CE, bool LO>
-__global__ void __launch_bounds__(kGemmThreads) egm_gemm_lds(
-    const float* __restrict__ S, int64_t T, int64_t stride, int N, int Kpad, int k_chunk,
-    const unsigned short* __restrict__ Wt, int Epad, float* __restrict__ ws, int mt, int nt,
-    int ncols) {
-  constexpr int NP = LO ? 2 : 1;           // planes
-  constexpr int kTW = 32 * RW;             // time rows per workgroup
-  constexpr int EW = 64 * CE;              // electrodes per workgroup
-  constexpr int ACH = kTW * 4 / kGemmThreads;      // 16-float sample chunks per thread
-  constexpr int BCH = NP * EW * 8 / kGemmThreads;  // 16-byte lead-field chunks per thread
-  static_assert(ACH >= 1 && BCH >= 1, "staging chunks");
-  __shared__ unsigned lds_a[NP][kTW * kPitch / 2];
-  __shared__ unsigned lds_b[NP][EW * kPitch / 2];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int tg = wave >> 2, wv = wave & 3;
-  const int slot = (int)(blockIdx.x >> 3);
-  const int col = (slot / mt) * 8 + (int)(blockIdx.x & 7);
-  if (col >= ncols) return;  // padding column (whole workgroup)
-  const int split = col / nt;
-  const int64_t tb = (int64_t)(slot % mt) * kTW;
-  const int eb = (col % nt) * EW;
-  const int kb = split * k_chunk;
-  const int ke = min(N, kb + k_chunk);
-  const int r16 = lane & 15, q = lane >> 4;
-
-  // sample staging: chunk c = tid + 512 i covers row c / 4, k offset 16 (c % 4) of the stage
-  int srw[ACH], sk[ACH];
-  const float* srow[ACH];
-#pragma unroll
-  for (int i = 0; i < ACH; ++i) {
-    const int c = tid + kGemmThreads * i;
-    srw[i] = c >> 2;
-    sk[i] = 16 * (c & 3);
-    int64_t t = tb + srw[i];
-    if (t > T - 1) t = T - 1;
-    srow[i] = S + t * stride;
-  }
-  // lead-field staging: chunk j = tid + 512 i -> plane, electrode row, k offset 8 (j % 8)
-  int bpl[BCH], brw[BCH], bk[BCH];
-#pragma unroll
-  for (int i = 0; i < BCH; ++i) {
-    const int j = tid + kGemmThreads * i;
-    bpl[i] = j / (EW * 8);
-    brw[i] = (j / 8) % EW;
-    bk[i] = 8 * (j % 8);
-  }
+// Staging registers of one K stage: KS-deep sample rows (FP32, split into BF16 hi/lo when
+// written to LDS) and lead-field rows (BF16 planes, copied as they are).
+template <int KTW, int EW, int NP, int KS>
+struct GemmStage {
+  static constexpr int SCH = KS / 16;                          // 16-float chunks per sample row
+  static constexpr int ACH = KTW * SCH / kGemmThreads;         // sample chunks per thread
+  static constexpr int BRC = KS / 8;                           // 16-byte chunks per lead-field row
+  static constexpr int BCH = NP * EW * BRC / kGemmThreads;     // lead-field chunks per thread
+  static constexpr int PA = KS + 8;  // LDS row pitch in bf16 (16 B pad against bank conflicts)
+  static_assert(ACH >= 1 && ACH * kGemmThreads == KTW * SCH, "sample staging");
+  static_assert(BCH >= 1 && BCH * kGemmThreads == NP * EW * BRC, "lead-field staging");
   float st[ACH][16];
   uint4 bst[BCH];
-  auto load_stage = [&](int k0) {
+  int srw[ACH], sk[ACH];   // sample chunk: row in the tile, k offset in the stage
+  const float* srow[ACH];  // its sample row (rows past T repeat row T - 1, never written)
+  int boff[BCH];           // lead-field chunk: element offset within a 64-deep Wt tile
+  int bl[BCH];             // its LDS word offset (plane, row, k)
+
+  __device__ __forceinline__ void init(int tid, const float* S, int64_t T, int64_t stride,
+                                       int64_t tb, int Epad, int eb) {
+#pragma unroll
+    for (int i = 0; i < ACH; ++i) {
+      const int c = tid + kGemmThreads * i;
+      srw[i] = c / SCH;
+      sk[i] = 16 * (c % SCH);
+      int64_t t = tb + srw[i];
+      if (t > T - 1) t = T - 1;
+      srow[i] = S + t * stride;
+    }
+#pragma unroll
+    for (int i = 0; i < BCH; ++i) {
+      const int j = tid + kGemmThreads * i;
+      const int pl = j / (EW * BRC), rw = (j / BRC) % EW, k = 8 * (j % BRC);
+      boff[i] = (pl * Epad + eb + rw) * kK2 + k;
+      bl[i] = pl * (EW * PA / 2) + (rw * PA + k) / 2;
+    }
+  }
+
+  // Loads the stage at k0 (a multiple of KS) into the staging registers; ke bounds the samples.
+  // The lead field is zero-padded to whole 64-deep tiles, so its loads are always in bounds.
+  __device__ __forceinline__ void load(int k0, int ke, const unsigned short* __restrict__ Wt,
+                                       int Epad) {
 #pragma unroll
     for (int i = 0; i < ACH; ++i) {
       const int k = k0 + sk[i];
@@ -280,59 +255,107 @@ __global__ void __launch_bounds__(kGemmThreads) egm_gemm_lds(
         for (int j = 0; j < 16; ++j) st[i][j] = k + j < ke ? srow[i][k + j] : 0.0f;
       }
     }
-    // lead field: zero-padded to Kpad, so whole stages are always in bounds; the stage's
-    // tile [plane][Epad][64] is contiguous
-    const unsigned short* tile = Wt + (size_t)(k0 / kK2) * 2 * Epad * kK2;
+    const unsigned short* tile = Wt + (size_t)(k0 / kK2) * 2 * Epad * kK2 + (k0 % kK2);
 #pragma unroll
-    for (int i = 0; i < BCH; ++i)
-      bst[i] = *(const uint4*)(tile + ((size_t)bpl[i] * Epad + eb + brw[i]) * kK2 + bk[i]);
-  };
+    for (int i = 0; i < BCH; ++i) bst[i] = *(const uint4*)(tile + boff[i]);
+  }
 
+  // Writes the staged stage to LDS: sample hi (and lo) planes, lead-field planes.
+  __device__ __forceinline__ void store(unsigned* __restrict__ la, unsigned* __restrict__ lb) const {
+#pragma unroll
+    for (int i = 0; i < ACH; ++i) {
+      unsigned h[8], l[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) split_pair(st[i][2 * j], st[i][2 * j + 1], h[j], l[j]);
+      const int o = (srw[i] * PA + sk[i]) / 2;
+      *(uint4*)&la[o] = uint4{h[0], h[1], h[2], h[3]};
+      *(uint4*)&la[o + 4] = uint4{h[4], h[5], h[6], h[7]};
+      if (NP == 2) {
+        *(uint4*)&la[KTW * PA / 2 + o] = uint4{l[0], l[1], l[2], l[3]};
+        *(uint4*)&la[KTW * PA / 2 + o + 4] = uint4{l[4], l[5], l[6], l[7]};
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < BCH; ++i) *(uint4*)&lb[bl[i]] = bst[i];
+  }
+};
+
+// egm partial (T x Epad, one K split), samples and lead field staged through LDS.
+// Workgroup = 8 waves: wave (tg, w) = (wave / 4, wave % 4) owns time rows
+// [tb + 16 RW tg, tb + 16 RW (tg + 1)) and electrodes [e0 + 16 CE w, e0 + 16 CE (w + 1)) as
+// RW x CE tiles of 16 x 16 with FP32 accumulators (workgroup: 32 RW time rows).  Per KS-deep
+// K stage the workgroup
+//   * reads its (32 RW) x KS FP32 sample tile from HBM once (4 KS B per time row) and splits it
+//     into BF16 hi/lo planes in LDS;
+//   * copies its (64 CE) x KS lead-field tile (BF16 hi/lo planes, L2/MALL-resident across
+//     time tiles) into LDS, once for both time groups;
+//   * runs every wave's MFMAs with both operands from LDS.
+// The next stage's tiles are loaded into registers while the current stage's MFMAs run.
+// LO: add the hi*lo and lo*hi products.
+// Operand layout of v_mfma_f32_16x16x32_bf16: lane (r16 = lane % 16, q = lane / 16) holds
+// A row r16 and B column r16, k = 8q .. 8q+7; the result lane holds column r16, rows 4q .. 4q+3.
+//
+// Tile order (1D grid, mt time tiles x ncols (electrode tile, K split) columns, the column count
+// padded to a multiple of 8): workgroups are dispatched to the 8 XCDs round-robin, so workgroup
+// id runs on XCD id % 8.  Slot id / 8 of XCD x walks the mt time tiles of column
+// 8 (slot / mt) + x, i.e. all time tiles of a column run on one XCD at about the same time
+// and read each lead-field stage once from HBM, then from that XCD's L2.
+template <int RW, int CE, bool LO, int KS>
+__global__ void __launch_bounds__(kGemmThreads) egm_gemm_lds(
+    const float* __restrict__ S, int64_t T, int64_t stride, int N, int Kpad, int k_chunk,
+    const unsigned short* __restrict__ Wt, int Epad, float* __restrict__ ws, int mt, int nt,
+    int ncols) {
+  constexpr int NP = LO ? 2 : 1;  // planes
+  constexpr int kTW = 32 * RW;    // time rows per workgroup
+  constexpr int EW = 64 * CE;     // electrodes per workgroup
+  using Stage = GemmStage<kTW, EW, NP, KS>;
+  constexpr int PA = Stage::PA;
+  __shared__ unsigned lds_a[NP * kTW * PA / 2];
+  __shared__ unsigned lds_b[NP * EW * PA / 2];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int tg = wave >> 2, wv = wave & 3;
+  const int slot = (int)(blockIdx.x >> 3);
+  const int col = (slot / mt) * 8 + (int)(blockIdx.x & 7);
+  if (col >= ncols) return;  // padding column (whole workgroup)
+  const int split = col / nt;
+  const int64_t tb = (int64_t)(slot % mt) * kTW;
+  const int eb = (col % nt) * EW;
+  const int kb = split * k_chunk;
+  const int ke = min(N, kb + k_chunk);
+  const int r16 = lane & 15, q = lane >> 4;
+
+  Stage stg;
+  stg.init(tid, S, T, stride, tb, Epad, eb);
   f32x4 acc[RW][CE];
 #pragma unroll
   for (int r = 0; r < RW; ++r)
 #pragma unroll
     for (int c = 0; c < CE; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  if (kb < ke) load_stage(kb);
-  for (int k0 = kb; k0 < ke; k0 += kK2) {
+  if (kb < ke) stg.load(kb, ke, Wt, Epad);
+  for (int k0 = kb; k0 < ke; k0 += KS) {
     __syncthreads();  // the previous stage's LDS reads are done
-    {
-#pragma unroll
-      for (int i = 0; i < ACH; ++i) {
-        unsigned h[8], l[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) split_pair(st[i][2 * j], st[i][2 * j + 1], h[j], l[j]);
-        const int o = (srw[i] * kPitch + sk[i]) / 2;
-        *(uint4*)&lds_a[0][o] = uint4{h[0], h[1], h[2], h[3]};
-        *(uint4*)&lds_a[0][o + 4] = uint4{h[4], h[5], h[6], h[7]};
-        if (LO) {
-          *(uint4*)&lds_a[NP - 1][o] = uint4{l[0], l[1], l[2], l[3]};
-          *(uint4*)&lds_a[NP - 1][o + 4] = uint4{l[4], l[5], l[6], l[7]};
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < BCH; ++i) *(uint4*)&lds_b[bpl[i]][(brw[i] * kPitch + bk[i]) / 2] = bst[i];
-    }
+    stg.store(lds_a, lds_b);
     __syncthreads();
-    if (k0 + kK2 < ke) load_stage(k0 + kK2);  // next stage in flight during the MFMAs
+    if (k0 + KS < ke) stg.load(k0 + KS, ke, Wt, Epad);  // next stage in flight during the MFMAs
 #pragma unroll
-    for (int s2 = 0; s2 < 2; ++s2) {
+    for (int s2 = 0; s2 < KS / 32; ++s2) {
       bf16x8 bh[CE], bl[CE];
 #pragma unroll
       for (int c = 0; c < CE; ++c) {
-        const int o = ((16 * CE * wv + 16 * c + r16) * kPitch + 32 * s2 + 8 * q) / 2;
-        bh[c] = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_b[0][o]);
-        if (LO) bl[c] = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_b[NP - 1][o]);
+        const int o = ((16 * CE * wv + 16 * c + r16) * PA + 32 * s2 + 8 * q) / 2;
+        bh[c] = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_b[o]);
+        if (LO) bl[c] = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_b[EW * PA / 2 + o]);
       }
       // per time block: the products of all CE electrode tiles, pass by pass, so that
       // consecutive MFMAs on one accumulator are CE apart
 #pragma unroll
       for (int r = 0; r < RW; ++r) {
-        const int o = ((16 * RW * tg + 16 * r + r16) * kPitch + 32 * s2 + 8 * q) / 2;
-        const bf16x8 ah = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_a[0][o]);
+        const int o = ((16 * RW * tg + 16 * r + r16) * PA + 32 * s2 + 8 * q) / 2;
+        const bf16x8 ah = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_a[o]);
         bf16x8 al;
-        if (LO) al = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_a[NP - 1][o]);
+        if (LO) al = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_a[kTW * PA / 2 + o]);
 #pragma unroll
         for (int c = 0; c < CE; ++c)
           acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[c], acc[r][c], 0, 0, 0);
@@ -359,6 +382,7 @@ __global__ void __launch_bounds__(kGemmThreads) egm_gemm_lds(
         if (t < T) wsp[t * Epad + e0 + 16 * c + r16] = acc[r][c][j];
       }
 }
+
 
 __global__ void egm_reduce(const float* __restrict__ ws, int splits, int64_t T, int Epad, int E,
                            float* __restrict__ out) {
@@ -752,12 +776,12 @@ int egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, float* 
     const bool lo = precision == 0;
     int& res = g->resident[lo ? 1 : 0];
     if (res == 0) {
-      const void* kern = g->ce == 4 ? (lo ? (const void*)egm_gemm_lds<rw_for(4), 4, true>
-                                          : (const void*)egm_gemm_lds<rw_for(4), 4, false>)
-                         : g->ce == 2 ? (lo ? (const void*)egm_gemm_lds<rw_for(2), 2, true>
-                                            : (const void*)egm_gemm_lds<rw_for(2), 2, false>)
-                                      : (lo ? (const void*)egm_gemm_lds<rw_for(1), 1, true>
-                                            : (const void*)egm_gemm_lds<rw_for(1), 1, false>);
+      const void* kern = g->ce == 4 ? (lo ? (const void*)egm_gemm_lds<rw_for(4), 4, true, ks_for(4)>
+                                          : (const void*)egm_gemm_lds<rw_for(4), 4, false, ks_for(4)>)
+                         : g->ce == 2 ? (lo ? (const void*)egm_gemm_lds<rw_for(2), 2, true, ks_for(2)>
+                                            : (const void*)egm_gemm_lds<rw_for(2), 2, false, ks_for(2)>)
+                                      : (lo ? (const void*)egm_gemm_lds<rw_for(1), 1, true, ks_for(1)>
+                                            : (const void*)egm_gemm_lds<rw_for(1), 1, false, ks_for(1)>);
       int per_cu = 0, dev = 0, cus = 0;
       PHS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kGemmThreads, 0));
       PHS_HIP_CHECK(hipGetDevice(&dev));
@@ -801,11 +825,11 @@ int egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, float* 
     };
     const bool lo = precision == 0;
     if (g->ce == 4)
-      lo ? launch(egm_gemm_lds<rw_for(4), 4, true>) : launch(egm_gemm_lds<rw_for(4), 4, false>);
+      lo ? launch(egm_gemm_lds<rw_for(4), 4, true, ks_for(4)>) : launch(egm_gemm_lds<rw_for(4), 4, false, ks_for(4)>);
     else if (g->ce == 2)
-      lo ? launch(egm_gemm_lds<rw_for(2), 2, true>) : launch(egm_gemm_lds<rw_for(2), 2, false>);
+      lo ? launch(egm_gemm_lds<rw_for(2), 2, true, ks_for(2)>) : launch(egm_gemm_lds<rw_for(2), 2, false, ks_for(2)>);
     else
-      lo ? launch(egm_gemm_lds<rw_for(1), 1, true>) : launch(egm_gemm_lds<rw_for(1), 1, false>);
+      lo ? launch(egm_gemm_lds<rw_for(1), 1, true, ks_for(1)>) : launch(egm_gemm_lds<rw_for(1), 1, false, ks_for(1)>);
   }
   const int64_t n = T * g->E;
   egm_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ws, splits, T, g->Epad, g->E, egm);

@@ -131,6 +131,7 @@ struct phs_arena {
     const void* pv = nullptr;  // PV captured in the kernel arguments
   } g[2];
   int32_t par = 0;         // launch parity of the next step (flips with every launched step)
+  uint32_t walk = 0;       // launches so far: odd ones walk the bricks backwards
   int32_t sweeps_eff = 2;  // sweeps of the first CN solve in the launched sequence
   int32_t flav = 0;        // kernel flavour (FLAV_*) the resident simulations allow
   // Launch timing of one eager step: while `marking`, every launch of the step is followed
@@ -237,10 +238,19 @@ struct Step {
 
   template <int FLAV>
   static void run_t(phs_arena* a, hipStream_t s) {
-    const ArenaDev A = dev_view(a);
+    const ArenaDev A0 = dev_view(a);
     const int grid = grid_for(a);
     const int grid1 = grid_for_nb1(a);
     const phs_arena_config& c = a->cfg;
+    // Launches alternate the brick walk (forward, backward, ...) across steps, so that each
+    // starts on the bricks the previous one finished with.  The walk changes no result: every
+    // launch computes each brick from the previous launch's buffers alone.
+    ArenaDev A = A0;
+    auto next = [&]() {
+      A = A0;
+      if (a->walk++ & 1) A.flags |= 4;
+    };
+    next();
     // the step's reaction already ran in the previous step's last sweep (or at add)
     k_rhs<BX, BY, BZ, NB, GUESS, FLAV><<<grid, kBlock, 0, s>>>(A);
     mark(a, s, KIND_RHS);
@@ -253,6 +263,7 @@ struct Step {
         const float* cur = k == 1 ? nullptr : bufs[k % 2];
         const float* prev = k <= 2 ? nullptr : bufs[(k - 1) % 2];
         const int prev_kind = k == 1 ? 0 : (k == 2 ? 1 : 2);
+        next();
         sweep<FLAV>(k < m - 1 ? MODE_MID : last_mode, grid, s, A, k, K, cur, prev, prev_kind,
                     bufs[(k + 1) % 2]);
         mark(a, s, !per_slot ? KIND_REFINE : k == 1 ? KIND_FIRST : k < m - 1 ? KIND_MID
@@ -264,6 +275,7 @@ struct Step {
     } else {
       solve(a->sweeps_eff, MODE_DACC_SET, true);
       for (int r = 0; r < c.refine; ++r) {
+        next();
         k_residual<BX, BY, BZ><<<grid1, kBlock, 0, s>>>(A);
         mark(a, s, KIND_RESIDUAL);
         solve(c.refine_sweeps, r == c.refine - 1 ? MODE_DACC_ADD_X : MODE_DACC_ADD, false);
