@@ -27,6 +27,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "phs_internal.h"
@@ -186,47 +187,67 @@ __device__ __forceinline__ void split_pair(float x0, float x1, unsigned& hi, uns
   lo = __builtin_bit_cast(unsigned, l);
 }
 
-constexpr int kGemmThreads = 512;    // 8 waves: 2 across time x 4 across electrodes
-// 16-row time blocks per wave (workgroup: 32 RW time rows) and K depth of one LDS stage:
-// RW 4, 64-deep stages for up to 128 electrodes (the sample stream dominates); for wider
-// electrode sets RW 8 (halves the lead-field reads per flop) with 32-deep stages, which keep
-// the 128 accumulator registers, the staged stage and the operands within the 256 registers
-// a wave has at 2 waves per SIMD (no spills).
-constexpr int rw_for(int ce) { return ce == 4 ? 8 : 4; }
-constexpr int ks_for(int ce) { return ce == 4 ? 32 : 64; }
+// Shape of the BF16 contraction per electrode-tile width CE (64 CE electrodes per workgroup):
+//   RW   16-row time blocks per wave, TG groups of 4 waves across time per workgroup
+//        (workgroup: 16 RW TG time rows), KS the K depth of one LDS stage;
+//   BLDS the lead-field tile is staged through LDS (shared by the TG time groups), else each
+//        wave reads its own lead-field fragments straight from the L2-resident tile.
+// Up to 128 electrodes: RW 4, TG 2, 64-deep stages, lead field through LDS (the sample stream
+// dominates).  256 electrodes: RW 8 (halves the lead-field reads per flop), 32-deep stages and
+// one time group whose waves read their lead-field fragments directly, which keeps the 128
+// accumulator registers, the staged samples and the operands within the 256 registers of a
+// wave at 2 waves per SIMD (no spills) and the LDS at 20 KB, so that two workgroups share a
+// CU and one's stage barrier and loads overlap the other's MFMAs.
+template <int CE>
+struct GemmCfg {
+  static constexpr int RW = CE == 4 ? 8 : 4;
+  static constexpr int TG = CE == 4 ? 1 : 2;
+  static constexpr int KS = CE == 4 ? 32 : 64;
+  static constexpr bool BLDS = CE != 4;
+  static constexpr int DEPTH = CE == 4 ? 2 : 1;  // stages of samples in flight per workgroup
+  static constexpr int THREADS = 256 * TG;
+  static constexpr int TW = 16 * RW * TG;  // time rows per workgroup
+};
 
 // Staging registers of one K stage: KS-deep sample rows (FP32, split into BF16 hi/lo when
 // written to LDS) and lead-field rows (BF16 planes, copied as they are).
-template <int KTW, int EW, int NP, int KS>
+template <int KTW, int EW, int NP, int KS, int THREADS, bool BLDS>
 struct GemmStage {
-  static constexpr int SCH = KS / 16;                          // 16-float chunks per sample row
-  static constexpr int ACH = KTW * SCH / kGemmThreads;         // sample chunks per thread
+  static constexpr int LPR = KS / 4;                           // float4 chunks per sample row
+  static constexpr int RPI = 64 / LPR;                         // sample rows per wave load
+  static constexpr int ACH = KTW * LPR / THREADS;              // sample chunks per thread
   static constexpr int BRC = KS / 8;                           // 16-byte chunks per lead-field row
-  static constexpr int BCH = NP * EW * BRC / kGemmThreads;     // lead-field chunks per thread
+  static constexpr int BCH = BLDS ? NP * EW * BRC / THREADS : 1;  // lead-field chunks per thread
   static constexpr int PA = KS + 8;  // LDS row pitch in bf16 (16 B pad against bank conflicts)
-  static_assert(ACH >= 1 && ACH * kGemmThreads == KTW * SCH, "sample staging");
-  static_assert(BCH >= 1 && BCH * kGemmThreads == NP * EW * BRC, "lead-field staging");
-  float st[ACH][16];
-  uint4 bst[BCH];
-  int srw[ACH], sk[ACH];   // sample chunk: row in the tile, k offset in the stage
-  const float* srow[ACH];  // its sample row (rows past T repeat row T - 1, never written)
+  static_assert(ACH >= 1 && ACH * THREADS == KTW * LPR, "sample staging");
+  static_assert(!BLDS || (BCH >= 1 && BCH * THREADS == NP * EW * BRC), "lead-field staging");
+  // Sample chunk i of a lane: 4 floats of row (wave ACH + i) RPI + lane / LPR at k offset
+  // 4 (lane % LPR), so that one wave load covers RPI whole KS-deep row segments.
+  // the staged values of one stage (a workgroup keeps DEPTH of them in flight)
+  struct Data {
+    float st[ACH][4];
+    uint4 bst[BCH];
+  };
+  int srw[ACH];            // sample chunk: row in the tile
+  int sk;                  // k offset of the lane's sample chunks in the stage
+  const float* srow[ACH];  // their sample rows (rows past T repeat row T - 1, never written)
   int boff[BCH];           // lead-field chunk: element offset within a 64-deep Wt tile
   int bl[BCH];             // its LDS word offset (plane, row, k)
 
   __device__ __forceinline__ void init(int tid, const float* S, int64_t T, int64_t stride,
                                        int64_t tb, int Epad, int eb) {
+    const int wave = tid >> 6, lane = tid & 63;
+    sk = 4 * (lane % LPR);
 #pragma unroll
     for (int i = 0; i < ACH; ++i) {
-      const int c = tid + kGemmThreads * i;
-      srw[i] = c / SCH;
-      sk[i] = 16 * (c % SCH);
+      srw[i] = (wave * ACH + i) * RPI + lane / LPR;
       int64_t t = tb + srw[i];
       if (t > T - 1) t = T - 1;
       srow[i] = S + t * stride;
     }
 #pragma unroll
     for (int i = 0; i < BCH; ++i) {
-      const int j = tid + kGemmThreads * i;
+      const int j = tid + THREADS * i;
       const int pl = j / (EW * BRC), rw = (j / BRC) % EW, k = 8 * (j % BRC);
       boff[i] = (pl * Epad + eb + rw) * kK2 + k;
       bl[i] = pl * (EW * PA / 2) + (rw * PA + k) / 2;
@@ -235,63 +256,70 @@ struct GemmStage {
 
   // Loads the stage at k0 (a multiple of KS) into the staging registers; ke bounds the samples.
   // The lead field is zero-padded to whole 64-deep tiles, so its loads are always in bounds.
-  __device__ __forceinline__ void load(int k0, int ke, const unsigned short* __restrict__ Wt,
-                                       int Epad) {
+  // FULL: the whole stage is in range (k0 + KS <= ke), known at the call site, so that the
+  // loop of full stages has no branch between these loads and their use (a branch merge makes
+  // the compiler's wait for earlier loads also wait for these).
+  template <bool FULL = false>
+  __device__ __forceinline__ void load(Data& d, int k0, int ke,
+                                       const unsigned short* __restrict__ Wt, int Epad) const {
+    auto& st = d.st;
+    auto& bst = d.bst;
+    const int k = k0 + sk;
+    if (FULL || k0 + KS <= ke) {  // whole stage in range (wave-uniform): plain 16-byte loads
 #pragma unroll
-    for (int i = 0; i < ACH; ++i) {
-      const int k = k0 + sk[i];
-      if (k + 16 <= ke) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          float4 x;
-          __builtin_memcpy(&x, srow[i] + k + 4 * v, 16);
-          st[i][4 * v] = x.x;
-          st[i][4 * v + 1] = x.y;
-          st[i][4 * v + 2] = x.z;
-          st[i][4 * v + 3] = x.w;
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < 16; ++j) st[i][j] = k + j < ke ? srow[i][k + j] : 0.0f;
+      for (int i = 0; i < ACH; ++i) {
+        float4 x;
+        __builtin_memcpy(&x, srow[i] + k, 16);
+        st[i][0] = x.x;
+        st[i][1] = x.y;
+        st[i][2] = x.z;
+        st[i][3] = x.w;
       }
-    }
-    const unsigned short* tile = Wt + (size_t)(k0 / kK2) * 2 * Epad * kK2 + (k0 % kK2);
+    } else {
 #pragma unroll
-    for (int i = 0; i < BCH; ++i) bst[i] = *(const uint4*)(tile + boff[i]);
+      for (int i = 0; i < ACH; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) st[i][j] = k + j < ke ? srow[i][k + j] : 0.0f;
+    }
+    if (BLDS) {
+      const unsigned short* tile = Wt + (size_t)(k0 / kK2) * 2 * Epad * kK2 + (k0 % kK2);
+#pragma unroll
+      for (int i = 0; i < BCH; ++i) bst[i] = *(const uint4*)(tile + boff[i]);
+    }
   }
 
   // Writes the staged stage to LDS: sample hi (and lo) planes, lead-field planes.
-  __device__ __forceinline__ void store(unsigned* __restrict__ la, unsigned* __restrict__ lb) const {
+  __device__ __forceinline__ void store(const Data& d, unsigned* __restrict__ la,
+                                        unsigned* __restrict__ lb) const {
+    const auto& st = d.st;
+    const auto& bst = d.bst;
 #pragma unroll
     for (int i = 0; i < ACH; ++i) {
-      unsigned h[8], l[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) split_pair(st[i][2 * j], st[i][2 * j + 1], h[j], l[j]);
-      const int o = (srw[i] * PA + sk[i]) / 2;
-      *(uint4*)&la[o] = uint4{h[0], h[1], h[2], h[3]};
-      *(uint4*)&la[o + 4] = uint4{h[4], h[5], h[6], h[7]};
-      if (NP == 2) {
-        *(uint4*)&la[KTW * PA / 2 + o] = uint4{l[0], l[1], l[2], l[3]};
-        *(uint4*)&la[KTW * PA / 2 + o + 4] = uint4{l[4], l[5], l[6], l[7]};
-      }
+      unsigned h0, l0, h1, l1;
+      split_pair(st[i][0], st[i][1], h0, l0);
+      split_pair(st[i][2], st[i][3], h1, l1);
+      const int o = (srw[i] * PA + sk) / 2;
+      *(uint2*)&la[o] = uint2{h0, h1};
+      if (NP == 2) *(uint2*)&la[KTW * PA / 2 + o] = uint2{l0, l1};
     }
+    if (BLDS)
 #pragma unroll
-    for (int i = 0; i < BCH; ++i) *(uint4*)&lb[bl[i]] = bst[i];
+      for (int i = 0; i < BCH; ++i) *(uint4*)&lb[bl[i]] = bst[i];
   }
 };
 
-// egm partial (T x Epad, one K split), samples and lead field staged through LDS.
-// Workgroup = 8 waves: wave (tg, w) = (wave / 4, wave % 4) owns time rows
+// egm partial (T x Epad, one K split), samples (and lead field) staged through LDS.
+// Workgroup = 4 TG waves: wave (tg, w) = (wave / 4, wave % 4) owns time rows
 // [tb + 16 RW tg, tb + 16 RW (tg + 1)) and electrodes [e0 + 16 CE w, e0 + 16 CE (w + 1)) as
-// RW x CE tiles of 16 x 16 with FP32 accumulators (workgroup: 32 RW time rows).  Per KS-deep
-// K stage the workgroup
-//   * reads its (32 RW) x KS FP32 sample tile from HBM once (4 KS B per time row) and splits it
-//     into BF16 hi/lo planes in LDS;
+// RW x CE tiles of 16 x 16 with FP32 accumulators.  Per KS-deep K stage the workgroup
+//   * reads its (16 RW TG) x KS FP32 sample tile from HBM once (4 KS B per time row, one wave
+//     load covering whole row segments) and splits it into BF16 hi/lo planes in LDS;
 //   * copies its (64 CE) x KS lead-field tile (BF16 hi/lo planes, L2/MALL-resident across
-//     time tiles) into LDS, once for both time groups;
-//   * runs every wave's MFMAs with both operands from LDS.
-// The next stage's tiles are loaded into registers while the current stage's MFMAs run.
-// LO: add the hi*lo and lo*hi products.
+//     time tiles) into LDS, once for all time groups (BLDS), or each wave reads its CE
+//     fragments per plane from that tile directly;
+//   * runs every wave's MFMAs with the sample operand from LDS.
+// The next stage's samples (and lead field) are loaded into registers while the current
+// stage's MFMAs run.  LO: add the hi*lo and lo*hi products.
 // Operand layout of v_mfma_f32_16x16x32_bf16: lane (r16 = lane % 16, q = lane / 16) holds
 // A row r16 and B column r16, k = 8q .. 8q+7; the result lane holds column r16, rows 4q .. 4q+3.
 //
@@ -300,18 +328,21 @@ struct GemmStage {
 // id runs on XCD id % 8.  Slot id / 8 of XCD x walks the mt time tiles of column
 // 8 (slot / mt) + x, i.e. all time tiles of a column run on one XCD at about the same time
 // and read each lead-field stage once from HBM, then from that XCD's L2.
-template <int RW, int CE, bool LO, int KS>
-__global__ void __launch_bounds__(kGemmThreads) egm_gemm_lds(
+template <int CE, bool LO>
+__global__ void __launch_bounds__(GemmCfg<CE>::THREADS, 2) egm_gemm_lds(
     const float* __restrict__ S, int64_t T, int64_t stride, int N, int Kpad, int k_chunk,
     const unsigned short* __restrict__ Wt, int Epad, float* __restrict__ ws, int mt, int nt,
     int ncols) {
+  using Cfg = GemmCfg<CE>;
+  constexpr int RW = Cfg::RW, KS = Cfg::KS;
+  constexpr bool BLDS = Cfg::BLDS;
   constexpr int NP = LO ? 2 : 1;  // planes
-  constexpr int kTW = 32 * RW;    // time rows per workgroup
+  constexpr int kTW = Cfg::TW;    // time rows per workgroup
   constexpr int EW = 64 * CE;     // electrodes per workgroup
-  using Stage = GemmStage<kTW, EW, NP, KS>;
+  using Stage = GemmStage<kTW, EW, NP, KS, Cfg::THREADS, BLDS>;
   constexpr int PA = Stage::PA;
   __shared__ unsigned lds_a[NP * kTW * PA / 2];
-  __shared__ unsigned lds_b[NP * EW * PA / 2];
+  __shared__ unsigned lds_b[BLDS ? NP * EW * PA / 2 : 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int tg = wave >> 2, wv = wave & 3;
@@ -324,8 +355,11 @@ __global__ void __launch_bounds__(kGemmThreads) egm_gemm_lds(
   const int kb = split * k_chunk;
   const int ke = min(N, kb + k_chunk);
   const int r16 = lane & 15, q = lane >> 4;
+  // direct lead-field fragments: row e = eb + 16 CE wv + 16 c + r16 of each plane's tile
+  const int bdir = (eb + 16 * CE * wv + r16) * kK2 + 8 * q;
+  const size_t bplane = (size_t)Epad * kK2;
 
-  Stage stg;
+  Stage stg;  // staging addresses
   stg.init(tid, S, T, stride, tb, Epad, eb);
   f32x4 acc[RW][CE];
 #pragma unroll
@@ -333,20 +367,46 @@ __global__ void __launch_bounds__(kGemmThreads) egm_gemm_lds(
 #pragma unroll
     for (int c = 0; c < CE; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  if (kb < ke) stg.load(kb, ke, Wt, Epad);
-  for (int k0 = kb; k0 < ke; k0 += KS) {
+  // Direct lead-field fragments (one 32-deep MFMA step per stage): those of a stage are
+  // loaded once the previous stage's MFMAs have consumed theirs, into the same registers, so
+  // that they arrive during the stage barrier and the sample store.  (Two register sets, each
+  // loaded a whole stage ahead, measured slower: the registers they take cost the samples
+  // their second stage in flight.)
+  static_assert(BLDS || KS == 32, "direct lead-field fragments: one MFMA step per stage");
+  struct Frags {
+    bf16x8 h[CE], l[CE];
+  };
+  auto load_b = [&](Frags& b, int k) {
+    const unsigned short* f = Wt + (size_t)(k / kK2) * 2 * bplane + (k % kK2) + bdir;
+#pragma unroll
+    for (int c = 0; c < CE; ++c) {
+      b.h[c] = __builtin_bit_cast(bf16x8, *(const uint4*)(f + 16 * c * kK2));
+      if (LO) b.l[c] = __builtin_bit_cast(bf16x8, *(const uint4*)(f + 16 * c * kK2 + bplane));
+    }
+  };
+  using Data = typename Stage::Data;
+  constexpr int DEPTH = Cfg::DEPTH;
+  // One stage at k0 from the staged values d: LDS store, the prefetch of stage k0 + DEPTH KS
+  // into d, the MFMAs, then the next stage's direct fragments into b.
+  // FULL: the prefetched stage and the next one are whole stages in range.
+  auto run_stage = [&](Data& d, Frags& b, int k0, auto full) {
+    constexpr bool FULL = decltype(full)::value;
     __syncthreads();  // the previous stage's LDS reads are done
-    stg.store(lds_a, lds_b);
+    stg.store(d, lds_a, lds_b);
     __syncthreads();
-    if (k0 + KS < ke) stg.load(k0 + KS, ke, Wt, Epad);  // next stage in flight during the MFMAs
+    if (FULL)
+      stg.template load<true>(d, k0 + DEPTH * KS, ke, Wt, Epad);
+    else if (k0 + DEPTH * KS < ke)
+      stg.load(d, k0 + DEPTH * KS, ke, Wt, Epad);
 #pragma unroll
     for (int s2 = 0; s2 < KS / 32; ++s2) {
-      bf16x8 bh[CE], bl[CE];
+      if (BLDS) {
 #pragma unroll
-      for (int c = 0; c < CE; ++c) {
-        const int o = ((16 * CE * wv + 16 * c + r16) * PA + 32 * s2 + 8 * q) / 2;
-        bh[c] = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_b[o]);
-        if (LO) bl[c] = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_b[EW * PA / 2 + o]);
+        for (int c = 0; c < CE; ++c) {
+          const int o = ((16 * CE * wv + 16 * c + r16) * PA + 32 * s2 + 8 * q) / 2;
+          b.h[c] = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_b[o]);
+          if (LO) b.l[c] = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_b[EW * PA / 2 + o]);
+        }
       }
       // per time block: the products of all CE electrode tiles, pass by pass, so that
       // consecutive MFMAs on one accumulator are CE apart
@@ -358,17 +418,39 @@ __global__ void __launch_bounds__(kGemmThreads) egm_gemm_lds(
         if (LO) al = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_a[kTW * PA / 2 + o]);
 #pragma unroll
         for (int c = 0; c < CE; ++c)
-          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[c], acc[r][c], 0, 0, 0);
+          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b.h[c], acc[r][c], 0, 0, 0);
         if (LO) {
 #pragma unroll
           for (int c = 0; c < CE; ++c)
-            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[c], acc[r][c], 0, 0, 0);
+            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b.l[c], acc[r][c], 0, 0, 0);
 #pragma unroll
           for (int c = 0; c < CE; ++c)
-            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[c], acc[r][c], 0, 0, 0);
+            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b.h[c], acc[r][c], 0, 0, 0);
         }
       }
     }
+    if (!BLDS && (FULL || k0 + KS < ke)) load_b(b, k0 + KS);
+  };
+  using Full = std::true_type;
+  using Part = std::false_type;
+  Data d0, d1;
+  Frags b;
+  Data& d_odd = DEPTH == 2 ? d1 : d0;
+  if (kb < ke) {
+    stg.load(d0, kb, ke, Wt, Epad);
+    if (DEPTH == 2 && kb + KS < ke) stg.load(d1, kb + KS, ke, Wt, Epad);
+    if (!BLDS) load_b(b, kb);
+  }
+  // stages in pairs: even ones from d0, odd ones from d_odd; pairs whose prefetches are all
+  // whole stages first, then the last few stages
+  int k0 = kb;
+  for (; k0 + (DEPTH + 2) * KS <= ke; k0 += 2 * KS) {
+    run_stage(d0, b, k0, Full());
+    run_stage(d_odd, b, k0 + KS, Full());
+  }
+  for (; k0 < ke; k0 += 2 * KS) {
+    run_stage(d0, b, k0, Part());
+    if (k0 + KS < ke) run_stage(d_odd, b, k0 + KS, Part());
   }
   float* wsp = ws + (size_t)split * T * Epad;
   const int e0 = eb + 16 * CE * wv;
@@ -382,7 +464,6 @@ __global__ void __launch_bounds__(kGemmThreads) egm_gemm_lds(
         if (t < T) wsp[t * Epad + e0 + 16 * c + r16] = acc[r][c][j];
       }
 }
-
 
 __global__ void egm_reduce(const float* __restrict__ ws, int splits, int64_t T, int Epad, int E,
                            float* __restrict__ out) {
@@ -761,7 +842,10 @@ int egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, float* 
     return phs::fail(PHS_ERR_ARG, "invalid electrogram run arguments");
   if (precision < 0 || precision > 2) return phs::fail(PHS_ERR_ARG, "precision must be 0, 1 or 2");
   const bool lds = precision != 1;
-  const int tm = lds ? 32 * rw_for(g->ce) : kTM;
+  const int tm = !lds ? kTM : g->ce == 4 ? GemmCfg<4>::TW : g->ce == 2 ? GemmCfg<2>::TW
+                                                                       : GemmCfg<1>::TW;
+  const int threads = g->ce == 4 ? GemmCfg<4>::THREADS : g->ce == 2 ? GemmCfg<2>::THREADS
+                                                                    : GemmCfg<1>::THREADS;
   const int te = lds ? 64 * g->ce : kTE;
   const int kstep = lds ? kK2 : kKS;
   const int mt = (int)((T + tm - 1) / tm);
@@ -776,21 +860,22 @@ int egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, float* 
     const bool lo = precision == 0;
     int& res = g->resident[lo ? 1 : 0];
     if (res == 0) {
-      const void* kern = g->ce == 4 ? (lo ? (const void*)egm_gemm_lds<rw_for(4), 4, true, ks_for(4)>
-                                          : (const void*)egm_gemm_lds<rw_for(4), 4, false, ks_for(4)>)
-                         : g->ce == 2 ? (lo ? (const void*)egm_gemm_lds<rw_for(2), 2, true, ks_for(2)>
-                                            : (const void*)egm_gemm_lds<rw_for(2), 2, false, ks_for(2)>)
-                                      : (lo ? (const void*)egm_gemm_lds<rw_for(1), 1, true, ks_for(1)>
-                                            : (const void*)egm_gemm_lds<rw_for(1), 1, false, ks_for(1)>);
+      const void* kern = g->ce == 4 ? (lo ? (const void*)egm_gemm_lds<4, true>
+                                          : (const void*)egm_gemm_lds<4, false>)
+                         : g->ce == 2 ? (lo ? (const void*)egm_gemm_lds<2, true>
+                                            : (const void*)egm_gemm_lds<2, false>)
+                                      : (lo ? (const void*)egm_gemm_lds<1, true>
+                                            : (const void*)egm_gemm_lds<1, false>);
       int per_cu = 0, dev = 0, cus = 0;
-      PHS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kGemmThreads, 0));
+      PHS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0));
       PHS_HIP_CHECK(hipGetDevice(&dev));
       PHS_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
       res = std::max(1, per_cu) * std::max(1, cus);
     }
     target = res;
   }
-  int splits = std::max(1, std::min(ksteps, (target + mt * nt - 1) / (mt * nt)));
+  // (at most the resident count: a partial second round of workgroups would double the time)
+  int splits = std::max(1, std::min(ksteps, target / (mt * nt)));
   const int k_chunk = (ksteps + splits - 1) / splits * kstep;
   splits = (g->N + k_chunk - 1) / k_chunk;
   const size_t need = sizeof(float) * (size_t)splits * T * g->Epad;
@@ -819,17 +904,17 @@ int egm_run(phs_egm* g, const float* samples, int64_t T, int64_t stride, float* 
                                       g->Epad, ws);
   } else {
     auto launch = [&](auto kern) {
-      kern<<<dim3(mt * ncols_pad), kGemmThreads, 0, s>>>(samples, T, stride, g->N, g->Kpad,
+      kern<<<dim3(mt * ncols_pad), threads, 0, s>>>(samples, T, stride, g->N, g->Kpad,
                                                          k_chunk, g->Wt, g->Epad, ws, mt, nt,
                                                          ncols);
     };
     const bool lo = precision == 0;
     if (g->ce == 4)
-      lo ? launch(egm_gemm_lds<rw_for(4), 4, true, ks_for(4)>) : launch(egm_gemm_lds<rw_for(4), 4, false, ks_for(4)>);
+      lo ? launch(egm_gemm_lds<4, true>) : launch(egm_gemm_lds<4, false>);
     else if (g->ce == 2)
-      lo ? launch(egm_gemm_lds<rw_for(2), 2, true, ks_for(2)>) : launch(egm_gemm_lds<rw_for(2), 2, false, ks_for(2)>);
+      lo ? launch(egm_gemm_lds<2, true>) : launch(egm_gemm_lds<2, false>);
     else
-      lo ? launch(egm_gemm_lds<rw_for(1), 1, true, ks_for(1)>) : launch(egm_gemm_lds<rw_for(1), 1, false, ks_for(1)>);
+      lo ? launch(egm_gemm_lds<1, true>) : launch(egm_gemm_lds<1, false>);
   }
   const int64_t n = T * g->E;
   egm_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ws, splits, T, g->Epad, g->E, egm);
