@@ -409,13 +409,18 @@ __global__ void __launch_bounds__(GemmCfg<CE>::THREADS, 2) egm_gemm_lds(
         }
       }
       // per time block: the products of all CE electrode tiles, pass by pass, so that
-      // consecutive MFMAs on one accumulator are CE apart
+      // consecutive MFMAs on one accumulator are CE apart; the next block's sample operands
+      // are read from LDS before this block's MFMAs issue
+      auto read_a = [&](int r, bf16x8& h, bf16x8& l) {
+        const int o = ((16 * RW * tg + 16 * r + r16) * PA + 32 * s2 + 8 * q) / 2;
+        h = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_a[o]);
+        if (LO) l = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_a[kTW * PA / 2 + o]);
+      };
+      bf16x8 ah, al, ahn, aln;
+      read_a(0, ah, al);
 #pragma unroll
       for (int r = 0; r < RW; ++r) {
-        const int o = ((16 * RW * tg + 16 * r + r16) * PA + 32 * s2 + 8 * q) / 2;
-        const bf16x8 ah = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_a[o]);
-        bf16x8 al;
-        if (LO) al = __builtin_bit_cast(bf16x8, *(const uint4*)&lds_a[kTW * PA / 2 + o]);
+        if (r + 1 < RW) read_a(r + 1, ahn, aln);
 #pragma unroll
         for (int c = 0; c < CE; ++c)
           acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b.h[c], acc[r][c], 0, 0, 0);
@@ -427,6 +432,8 @@ __global__ void __launch_bounds__(GemmCfg<CE>::THREADS, 2) egm_gemm_lds(
           for (int c = 0; c < CE; ++c)
             acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b.h[c], acc[r][c], 0, 0, 0);
         }
+        ah = ahn;
+        al = aln;
       }
     }
     if (!BLDS && (FULL || k0 + KS < ke)) load_b(b, k0 + KS);
