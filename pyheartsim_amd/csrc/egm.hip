@@ -494,8 +494,8 @@ __global__ void egm_reduce(const float* __restrict__ ws, int splits, int64_t T, 
 // contiguous bytes) and the samples split into hi/lo in registers: no LDS and no barriers.
 // At T = 1 the work is bound by the lead-field stream (4 bytes per electrode and voxel), not
 // by the MFMAs.  Each item writes its partial (T x 64) block to the workspace; a second kernel
-// sums the chunks of each output element in chunk order, so the result depends only on the
-// job itself, not on the other jobs of the launch.
+// sums the chunks of each output element in a fixed order (egm_stream_reduce_k), so the result
+// depends only on the job itself, not on the other jobs of the launch.
 constexpr int kStreamJobs = 64;
 constexpr int kStreamRows = 16;    // time rows of one job (one MFMA row block)
 constexpr int kStreamStages = 16;  // 64-deep K stages per work item (1024 voxels)
@@ -508,7 +508,7 @@ struct StreamJob {
   int64_t stride;
   int32_t T, N, Epad, E;
   int32_t item0;  // first work item of the job
-  int32_t out0;   // first output element of the job (reduction)
+  int32_t tile0;  // first (time row, 64-electrode tile) of the job (reduction)
 };
 struct StreamTable {
   int32_t n;
@@ -590,22 +590,36 @@ __global__ void __launch_bounds__(64 * kStreamWaves) egm_stream_k(const StreamTa
     }
 }
 
-__global__ void egm_stream_reduce_k(const StreamTable tab, const float* __restrict__ ws,
-                                    int n_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_out) return;
+// Sum of the chunk partials of one (job, time row, 64-electrode tile) per workgroup: wave w
+// adds chunks w, w + 16, ... in order, then the 16 partial sums are added in wave order, so the
+// result depends only on the job itself.
+constexpr int kReduceWaves = 16;
+__global__ void __launch_bounds__(64 * kReduceWaves) egm_stream_reduce_k(const StreamTable tab,
+                                                                         const float* __restrict__ ws) {
+  __shared__ float part[kReduceWaves][64];
+  const int tile = blockIdx.x;
   int j = 0;
   for (int q = 1; q < tab.n; ++q)
-    if (tab.j[q].out0 <= i) j = q;
+    if (tab.j[q].tile0 <= tile) j = q;
   const StreamJob& J = tab.j[j];
-  const int li = i - J.out0;
-  const int t = li / J.E, e = li % J.E;
   const int etiles = J.Epad >> 6;
+  const int li = tile - J.tile0;
+  const int t = li / etiles, et = li % etiles;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int chunks = stream_chunks(J.N);
-  const float* p = ws + ((size_t)(J.item0 + (e >> 6)) * kStreamRows + t) * 64 + (e & 63);
+  const float* p = ws + ((size_t)(J.item0 + et) * kStreamRows + t) * 64 + lane;
+  const size_t cstride = (size_t)etiles * kStreamRows * 64;
   float sum = 0.0f;
-  for (int c = 0; c < chunks; ++c) sum += p[(size_t)c * etiles * kStreamRows * 64];
-  J.out[li] = sum;
+  for (int c = w; c < chunks; c += kReduceWaves) sum += p[c * cstride];
+  part[w][lane] = sum;
+  __syncthreads();
+  if (w == 0) {
+    float tot = part[0][lane];
+#pragma unroll
+    for (int k = 1; k < kReduceWaves; ++k) tot += part[k][lane];
+    const int e = et * 64 + lane;
+    if (e < J.E) J.out[(size_t)t * J.E + e] = tot;
+  }
 }
 
 constexpr int kNormThreads = 1024;
@@ -1028,7 +1042,7 @@ int phs_egm_stream(const phs_egm_job* jobs, int32_t n_jobs, void* workspace,
   for (int32_t g0 = 0; g0 < n_jobs; g0 += kStreamJobs) {
     StreamTable tab;
     std::memset(&tab, 0, sizeof tab);
-    int32_t n_items = 0, n_out = 0;
+    int32_t n_items = 0, n_tiles = 0;
     for (int32_t i = g0; i < std::min(n_jobs, g0 + kStreamJobs); ++i) {
       const phs_egm_job& J = jobs[i];
       const phs_egm* g = J.lead_field;
@@ -1042,13 +1056,13 @@ int phs_egm_stream(const phs_egm_job* jobs, int32_t n_jobs, void* workspace,
       d.Epad = g->Epad;
       d.E = g->E;
       d.item0 = n_items;
-      d.out0 = n_out;
+      d.tile0 = n_tiles;
       n_items += host_stream_chunks(g->N) * (g->Epad / 64);
-      n_out += d.T * g->E;
+      n_tiles += d.T * (g->Epad / 64);
     }
     egm_stream_k<<<(n_items + kStreamWaves - 1) / kStreamWaves, 64 * kStreamWaves, 0, s>>>(
         tab, ws, n_items);
-    egm_stream_reduce_k<<<(n_out + 255) / 256, 256, 0, s>>>(tab, ws, n_out);
+    egm_stream_reduce_k<<<n_tiles, 64 * kReduceWaves, 0, s>>>(tab, ws);
     PHS_HIP_CHECK(hipGetLastError());
   }
   return PHS_OK;
