@@ -52,7 +52,8 @@ METRIC = "voxel-steps/s at LAT err<=1dt (chosen vs plain FP64); p50/p99 sim late
 PROFILES = os.path.join(ROOT, "profiles")
 PROFILE_TRAFFIC = (os.path.join(PROFILES, "arena_sweep_pmc_batched64.md"),
                    os.path.join(PROFILES, "arena_sweep_pmc_shell_ms_x16.md"))
-PROFILE_TRAFFIC_EGM = os.path.join(PROFILES, "egm_shell_e64_t2000.md")
+PROFILE_TRAFFIC_EGM = (os.path.join(PROFILES, "egm_shell_e64_t2000.md"),
+                       os.path.join(PROFILES, "egm_shell_e256_t2048.md"))
 
 
 def log(*a):
@@ -80,6 +81,17 @@ def profiled_traffic(cnt):
     for path in PROFILE_TRAFFIC:
         r = _profile_record(path, {"voxels": cnt["voxels"], "bricks": cnt["bricks"],
                                    "record_bytes": BRICK_RECORD_BYTES}, "sweep_bytes_per_launch")
+        if r is not None:
+            return r
+    return None
+
+
+def egm_traffic(n_voxel, electrodes, samples):
+    """HBM bytes of one electrogram launch, when a committed summary was measured on the same
+    contraction shape."""
+    for path in PROFILE_TRAFFIC_EGM:
+        r = _profile_record(path, {"n_voxel": n_voxel, "electrodes": electrodes,
+                                   "samples": samples}, "bytes_per_launch")
         if r is not None:
             return r
     return None
@@ -670,9 +682,7 @@ def run_egm(args, torch, runtime, pre):
              "frac": gbs / HBM_PEAK_GBS})
     roof.update(kernel="egm_gemm_lds + egm_reduce (one electrogram)", bytes_per_launch=nbytes,
                 hbm_gbs=gbs, mfma_issued_tflops=issued,
-                traffic=_profile_record(PROFILE_TRAFFIC_EGM,
-                                        {"n_voxel": N, "electrodes": E, "samples": T},
-                                        "bytes_per_launch"))
+                traffic=egm_traffic(N, E, T))
     return {
         "metric": "electrogram useful TFLOP/s (2 E T N per launch, BF16 hi/lo MFMA)",
         "value": useful / (ms * 1e-3) / 1e12, "unit": "TFLOP/s",

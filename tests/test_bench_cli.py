@@ -45,11 +45,8 @@ def test_profiled_traffic_matches_measured_arena_only():
 
 
 def test_profiled_traffic_of_the_electrogram():
-    rec = bench._profile_record(bench.PROFILE_TRAFFIC_EGM,
-                                {"n_voxel": 199446, "electrodes": 64, "samples": 2000},
-                                "bytes_per_launch")
-    algorithmic = 4.0 * 2000 * 199446 + 4.0 * 64 * 199446 + 4.0 * 2000 * 64
-    assert rec is not None and algorithmic < rec < 1.3 * algorithmic
-    assert bench._profile_record(bench.PROFILE_TRAFFIC_EGM,
-                                 {"n_voxel": 199446, "electrodes": 256, "samples": 2000},
-                                 "bytes_per_launch") is None
+    for n, e, t in ((199446, 64, 2000), (1596158, 256, 2048)):
+        rec = bench.egm_traffic(n, e, t)
+        algorithmic = 4.0 * t * n + 4.0 * e * n + 4.0 * t * e
+        assert rec is not None and algorithmic < rec < 1.3 * algorithmic
+    assert bench.egm_traffic(199446, 256, 2000) is None
