@@ -58,8 +58,8 @@ def test_noalloc_rejects_requests_beyond_the_scratch(cuda):
     N = g["voxel"].shape[0]
     el = np.random.default_rng(1).uniform(-5, 5, size=(256, 3))
     lf = LeadField(el, g["voxel"], None, w["hm"]["c_voxel"], g["Delta"], g["neighbor_id_2d"])
-    # 20000 samples x 256 electrodes: at least 2 K splits x 20000 x 256 x 4 B > 64 MiB
-    T = 20000
+    # 70000 samples x 256 electrodes x 4 B > 64 MiB of workspace even with a single K split
+    T = 70000
     s = torch.zeros((T, N), device="cuda")
     out = torch.zeros((T, 256), device="cuda")
     rc = _run(lib, lib.phs_egm_run_noalloc, lf, s, out, 0)
