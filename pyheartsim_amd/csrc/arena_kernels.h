@@ -113,6 +113,14 @@ struct ArenaDev {
   int32_t max_sims;  // slots to advance
 };
 
+// Record prefetch (flags bits 8-23: a distance in bricks, 0 = none).  Each wavefront loads
+// the record of the brick that distance ahead in its walk, which a wavefront of a later round
+// of resident wavefronts processes, so that wavefront's record load hits L2 instead of waiting
+// on HBM.  The loaded value is kept alive to the end of the kernel (keep_alive), whose
+// s_endpgm waits for every outstanding memory operation anyway.
+__device__ __forceinline__ void keep_alive(uint32_t v) { asm volatile("" ::"v"(v)); }
+
+
 // The brick-order bits of flags (XCD remap, backwards walk) for first_brick.
 __device__ __forceinline__ int xcd_of(const ArenaDev& A) { return A.flags & 5; }
 __device__ __forceinline__ int par_of(const ArenaDev& A) { return (A.flags >> 1) & 1; }
@@ -130,6 +138,17 @@ __device__ __forceinline__ T ld(const T* base, int i) {
 template <typename T>
 __device__ __forceinline__ void stv(T* base, int i, T v) {
   *(T*)((char*)base + (uint32_t)i * (uint32_t)sizeof(T)) = v;
+}
+
+template <int NB>
+__device__ __forceinline__ uint32_t prefetch_record(const ArenaDev& A, int b0) {
+  const int dist = (A.flags >> 8) & 0xffff;
+  const int t = (A.flags & 4) ? b0 - dist : b0 + dist;
+  const int l = (int)(threadIdx.x & 63);
+  uint32_t v = 0;
+  if (dist != 0 && t >= 0 && t + NB <= A.n_bricks && l < 48 * NB)
+    v = ((const uint32_t*)(A.bricks + t))[l];
+  return v;
 }
 
 // Warm start of the CN correction d (x = u* + d): 0, the previous step's correction d_{n-1}
@@ -536,6 +555,7 @@ __device__ __forceinline__ void advance_slot(SlotRec& S, int par) {
 template <int BX, int BY, int BZ, int NB, int GUESS, int FLAV>
 __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
   const int b0 = first_brick<NB>(xcd_of(A));
+  const uint32_t pfr = prefetch_record<NB>(A, b0);
   const int lane = lane_id();
   Stencil<BX, BY, BZ> st[NB];
   SlotView sv[NB];
@@ -580,6 +600,7 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
     const float a = alpha_at<float, FLAV>(sv[j], ain[j]);
     stv(A.R0, st[j].idx, 2.0f * a * s - (gn - a * sg));
   }
+  keep_alive(pfr);
   // The first workgroup's first wavefront also stages every slot's next-step values.  This
   // comes after all of the kernel's loads: a store to the slot table before them would leave
   // the compiler unable to keep the brick records on the scalar path.
@@ -630,6 +651,7 @@ template <int BX, int BY, int BZ, int MODE, int NB, int GUESS, bool FIRST, int F
 __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, const float* cur,
                                                   const float* prev, int prev_kind, float* out) {
   const int b0 = first_brick<NB>(xcd_of(A));
+  const uint32_t pfr = prefetch_record<NB>(A, b0);
   const int lane = lane_id();
   const float* src = FIRST ? A.R0 : cur;
   Stencil<BX, BY, BZ> st[NB];
@@ -664,7 +686,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       const int i = st[j].idx;
       dn[j] = ld(src, i);
       r0[j] = ld(A.R0, i);
-      dp[j] = prev_kind == 2 ? ld(prev, i) : 0.0f;
+      dp[j] = !FIRST && prev_kind == 2 ? ld(prev, i) : 0.0f;
       st[j].gather(src, g6[j]);
       ain[j].template load<FLAV>(A, sv[j], i, br);
       if (MODE == MODE_DACC_ADD || MODE == MODE_DACC_ADD_X) pd[j] = A.DACC[i];
@@ -697,7 +719,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
     const int i = st[j].idx;
     const float a = alpha_at<float, FLAV>(sv[j], ain[j]);
     const float Ad = d - a * s;
-    const float dpv = prev_kind == 1 ? r0[j] * inv_theta : dp[j];
+    const float dpv = FIRST ? 0.0f : prev_kind == 1 ? r0[j] * inv_theta : dp[j];
     const float dnew = sv[j].ident ? d : sv[j].ca * d + sv[j].cb * dpv + sv[j].cg * (r0[j] - Ad);
     if (MODE == MODE_MID) {
       stv(out, i, dnew);
@@ -727,6 +749,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
       finish_voxel(A, sf[j], i, (float)((double)us + acc), pf[j], ph[j], pp[j]);
     }
   }
+  // s_endpgm waits for every outstanding memory operation, so this wait costs nothing extra
+  keep_alive(pfr);
 }
 
 // FP64 residual of the accumulated correction: r = 2 alpha sum(u*_m - u*_n) - A dacc.

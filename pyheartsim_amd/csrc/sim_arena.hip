@@ -160,6 +160,16 @@ void mark(phs_arena* a, hipStream_t s, int kind) {
   ++a->n_marks;
 }
 
+int nb_per_wave(const phs_arena* a);
+
+// Record prefetch distance (bricks ahead in a wavefront's walk): one round of resident
+// wavefronts per XCD (32 CUs x 32 waves), measured flat from 512 to 2048 bricks.  Arenas of
+// fewer than 16 rounds' bricks stay in L2 and the memory-side cache anyway, so they skip it.
+int prefetch_distance(const phs_arena* a) {
+  const int dist = 1024 * nb_per_wave(a);
+  return a->brick_high >= 16 * (int64_t)dist ? dist : 0;
+}
+
 ArenaDev dev_view(const phs_arena* a) {
   ArenaDev A;
   A.bricks = a->bricks;
@@ -178,7 +188,7 @@ ArenaDev dev_view(const phs_arena* a) {
   for (int k = 0; k < 5; ++k) A.PV[k] = a->PV[k];
   A.NONFINITE = a->NONFINITE;
   A.n_bricks = (int32_t)a->brick_high;
-  A.flags = (a->cfg.xcd_remap ? 1 : 0) | (a->par << 1);
+  A.flags = (a->cfg.xcd_remap ? 1 : 0) | (a->par << 1) | (prefetch_distance(a) << 8);
   A.diag = a->cfg.diag ? 1 : 0;
   A.max_sims = a->cfg.max_sims;
   return A;
