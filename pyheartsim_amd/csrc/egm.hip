@@ -674,9 +674,18 @@ __global__ void __launch_bounds__(kNormThreads) egm_normalize_k(float* __restric
 
 // Electrode activation time from the electrogram (utility/lat_map.py:20-27): per electrode
 // the first time index of the steepest downstroke, argmax_t -(egm[t+1] - egm[t]), then
-// shifted so that the earliest electrode is 0.  One workgroup per electrode; ties go to the
-// lower index (NumPy argmax).
+// shifted so that the earliest electrode is 0.  One workgroup per electrode; the order is
+// NumPy argmax's: a NaN is above every number (the first NaN wins), and ties go to the lower
+// index.
 constexpr int kLatThreads = 256;
+
+// (v, i) comes before (w, j) in argmax order.
+__device__ __forceinline__ bool argmax_before(float v, int64_t i, float w, int64_t j) {
+  const bool nv = __builtin_isnan(v), nw = __builtin_isnan(w);
+  if (nv || nw) return nv && (!nw || i < j);
+  return v > w || (v == w && i < j);
+}
+
 __global__ void __launch_bounds__(kLatThreads) egm_downstroke_k(const float* __restrict__ egm,
                                                                 int64_t T, int E,
                                                                 int32_t* __restrict__ idx) {
@@ -687,7 +696,7 @@ __global__ void __launch_bounds__(kLatThreads) egm_downstroke_k(const float* __r
   int64_t bi = T;  // sentinel above every index
   for (int64_t t = tid; t + 1 < T; t += kLatThreads) {
     const float v = -(egm[(t + 1) * E + e] - egm[t * E + e]);
-    if (v > best) {  // strided: a later t of the same thread never wins a tie
+    if (argmax_before(v, t, best, bi)) {
       best = v;
       bi = t;
     }
@@ -699,7 +708,7 @@ __global__ void __launch_bounds__(kLatThreads) egm_downstroke_k(const float* __r
     if (tid < o) {
       const float v = sv[tid + o];
       const int64_t i = si[tid + o];
-      if (v > sv[tid] || (v == sv[tid] && i < si[tid])) {
+      if (argmax_before(v, i, sv[tid], si[tid])) {
         sv[tid] = v;
         si[tid] = i;
       }

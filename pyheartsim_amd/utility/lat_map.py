@@ -6,7 +6,7 @@ result is the per-electrode index of the steepest downstroke,
 argmax_t -(s[t+1] - s[t]) (first index on ties), minus its minimum, as int64
 (E,).  It runs as two small kernels (phs_egm_electrode_lat) on the electrogram
 where it already is on the device, so no (T x E) host round trip is needed.
-NaN samples never win (the reference's NumPy argmax would return the first NaN).
+NaN differences order as NumPy's argmax orders them: the first NaN wins.
 
 Keyword-only: ``device_out=True`` returns the int32 torch tensor.
 """

@@ -70,6 +70,28 @@ def test_electrode_lat_matches_oracle(cuda, T, E):
     assert np.array_equal(got, want)
 
 
+def test_electrode_lat_nan_and_infinite_samples(cuda):
+    """NumPy's argmax puts a NaN above every number (the first NaN wins) and takes the first
+    of equal values, -inf included; the device scan follows the same order."""
+    from oracle import egm_oracle
+    from pyheartsim_amd.utility.lat_map import compute_electrode_lat
+    rng = np.random.default_rng(5)
+    T, E = 700, 9
+    s = np.cumsum(rng.standard_normal((T, E)), axis=0).astype(np.float32)
+    s[400, 0] = np.nan                     # differences 399 and 400 are NaN: 399 wins
+    s[[10, 650], 1] = np.nan               # the earliest NaN difference (9) wins
+    s[:, 2] = np.inf                       # every difference is NaN: index 0
+    s[300:, 3] = -np.inf                   # one +inf downstroke at 299, then NaN from 300
+    s[:, 4] = -np.inf                      # all NaN
+    s[5, 5] = np.inf                       # +inf at t = 5: differences of -inf and +inf
+    s[:, 6] = 0.0                          # constant: all ties, index 0
+    s[:200, 7] = np.linspace(0.0, -1e38, 200, dtype=np.float32)  # finite, steep
+    got = compute_electrode_lat(s)
+    with np.errstate(invalid="ignore"):
+        want = egm_oracle.electrode_lat(s)
+    assert np.array_equal(got, want), (got, want)
+
+
 def test_electrode_lat_of_simulated_egm(cuda):
     from oracle import egm_oracle
     from pyheartsim_amd.simulation import unipolar_electrogram_gpu
