@@ -108,29 +108,31 @@ const char* phs_last_error(void);
 int phs_device_count(int32_t* count);
 
 /* Host only.  Time-step stability of the inexact CN step of the bricked arena: with
- * spec(I - dt/2 L) in [1, lambda_max], a Chebyshev solve of `sweeps` sweeps (plus `refine`
- * FP64-residual passes of `refine_sweeps`) started from warm start `guess` (0, 1, 2) grows
- * the stiffest modes by *amplification per step (> 1 = unstable; the exact solve gives the
- * CN factor, <= 1).  phs_stable_sweeps gives the fewest sweeps with amplification <= 1
- * (*stable) and the arena's automatic count (*automatic, may be NULL): the fewest stable
- * sweeps whose error bound at the stiffest mode, lambda_max * max|P|, is <= 1/4 and under
- * which every stiff mode (lambda >= 2) shrinks by e within 40 steps (amplification over
- * [2, lambda_max] <= 1 - 1/40) and whose bound max|P| itself is <= 0.03 (the accuracy
- * target at fine time steps; see phs_auto_sweeps for coarse ones).
+ * spec(I - dt/2 L) in [1, lambda_max], a Chebyshev solve of `sweeps` sweeps on [1, lambda_max]
+ * (plus `refine` FP64-residual passes of `refine_sweeps`) started from warm start `guess`
+ * (0, 1, 2) grows the stiffest modes by *amplification per step (> 1 = unstable; the exact
+ * solve gives the CN factor, <= 1).  phs_stable_sweeps gives the fewest such sweeps with
+ * amplification <= 1 (*stable, the least fixed count the arena accepts) and the arena's
+ * automatic count at fine time steps (*automatic, may be NULL; see phs_auto_sweeps).
  * No counterpart in the reference, whose BiCGSTAB solve runs to a tolerance. */
 int phs_stable_sweeps(double lambda_max, int32_t guess, int32_t refine, int32_t refine_sweeps,
                       int32_t* stable, int32_t* automatic);
 int phs_step_amplification(double lambda_max, int32_t guess, int32_t sweeps, int32_t refine,
                            int32_t refine_sweeps, double* amplification);
 /* Host only.  The arena's automatic sweep count for one simulation (model and params as in
- * phs_sim_desc, dt its step): the automatic count of phs_stable_sweeps, raised where needed so
- * that the Chebyshev error bound (times the refinement passes' bounds) stays below
- * 0.03 min(1, (0.025 / s)^2), s = dt times the cell model's steepest upstroke rate
- * (*resolution, may be NULL): coarse time steps leave the warm start less to predict.
+ * phs_sim_desc, dt its step), with s = dt times the cell model's steepest upstroke rate
+ * (*resolution, may be NULL): the fewest sweeps m, on a Chebyshev interval [lo, lambda_max]
+ * (*lambda_lo, may be NULL), such that the step is stable, every stiff mode (lambda >= 2)
+ * shrinks by e within 40 steps (amplification over [2, lambda_max] <= 1 - 1/40), the error
+ * bound max|P| over the interval (times the refinement passes' bounds) is <= 1/(4 lambda_max)
+ * and <= 0.03 min(1, (0.025 / s)^2) (coarse time steps leave the warm start less to predict),
+ * and, for lo > 1, the smoothest mode's error |P(1)| is <= 0.16.  lo > 1 is considered only
+ * at fine steps (s <= 0.025) without refinement and for lambda_max <= 10 (the spectra where
+ * it was validated); it damps the stiff modes with fewer sweeps.
  * No counterpart in the reference, whose solver runs to a tolerance. */
 int phs_auto_sweeps(double lambda_max, int32_t guess, int32_t refine, int32_t refine_sweeps,
                     int32_t model, const double* params, double dt, int32_t* sweeps,
-                    double* resolution);
+                    double* resolution, double* lambda_lo);
 
 /* Host only (no device work): the Morton brick plan of a voxel set for brick shape
  * brick[3] (bx*by*bz = 64).  perm [n_voxel] receives the compact -> caller order,

@@ -98,7 +98,7 @@ SIGNATURES = {
     "phs_device_count": (C.c_int, [_PI32]),
     "phs_stable_sweeps": (C.c_int, [_D, _I32, _I32, _I32, _PI32, _PI32]),
     "phs_step_amplification": (C.c_int, [_D, _I32, _I32, _I32, _I32, _PD]),
-    "phs_auto_sweeps": (C.c_int, [_D, _I32, _I32, _I32, _I32, _P, _D, _PI32, _PD]),
+    "phs_auto_sweeps": (C.c_int, [_D, _I32, _I32, _I32, _I32, _P, _D, _PI32, _PD, _PD]),
     "phs_plan_build": (C.c_int, [_I32, _P, _P, _I32, _P, _P, _PI32, _P]),
     "phs_plain_create":(C.c_int, [C.POINTER(SimDesc), C.POINTER(Outputs), _D, C.POINTER(_P)]),
     "phs_plain_run": (C.c_int, [_P, _I64, _P]),

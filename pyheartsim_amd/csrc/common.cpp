@@ -1,7 +1,10 @@
 // Error reporting, version, Chebyshev coefficients.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -63,13 +66,23 @@ static double max_root_modulus(int n, const double* c) {
   return r;
 }
 
+// The largest root modulus over n + 1 grid points of [lam_lo, lmax] (n divides 4096, so the
+// grid is a subset of the public function's and its maximum a lower bound of that one).
+static double amplification_grid(double lmax, int guess, int sweeps, int refine,
+                                 int refine_sweeps, double lam_lo, double cheb_lo, int n);
+
 double step_amplification(double lmax, int guess, int sweeps, int refine, int refine_sweeps,
-                          double lam_lo) {
+                          double lam_lo, double cheb_lo) {
+  return amplification_grid(lmax, guess, sweeps, refine, refine_sweeps, lam_lo, cheb_lo, 4096);
+}
+
+static double amplification_grid(double lmax, int guess, int sweeps, int refine,
+                                 int refine_sweeps, double lam_lo, double cheb_lo, int n) {
   if (!(lmax > 1.0)) return 1.0;
   if (!(lam_lo < lmax)) return 0.0;
   const int kmax = std::max(sweeps, refine_sweeps);
   std::vector<ChebCoef> cf(std::max(kmax, 1));
-  chebyshev_coefficients(1.0, lmax, kmax, cf.data());
+  chebyshev_coefficients(cheb_lo, lmax, kmax, cf.data());
   // residual polynomial of m sweeps from the recurrence, evaluated at lam
   auto poly = [&](double lam, int m) {
     double ep = 1.0, e = 1.0 - cf[0].g * lam;
@@ -80,7 +93,6 @@ double step_amplification(double lmax, int guess, int sweeps, int refine, int re
     }
     return e;
   };
-  const int n = 4096;
   double worst = 0.0;
   for (int i = 0; i <= n; ++i) {
     const double lam = lam_lo + (lmax - lam_lo) * i / n;
@@ -117,14 +129,92 @@ static double cheb_bound(double lmax, int m) {
   return 2.0 * qm / (1.0 + qm * qm);
 }
 
+// Chebyshev polynomial T_m at x (any real x).
+static double cheb_t(int m, double x) {
+  if (std::fabs(x) <= 1.0) return std::cos(m * std::acos(x));
+  const double c = std::cosh(m * std::acosh(std::fabs(x)));
+  return (x < 0.0 && (m & 1)) ? -c : c;
+}
+
+static int auto_sweeps_uncached(double lmax, int guess, int refine, int refine_sweeps,
+                                int kmax, double resolution, double* cheb_lo);
+
+// Simulations of one voxel size and time step share a spectral bound, so the (count,
+// interval) of an input is computed once per process.
 int auto_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax,
-                double resolution) {
+                double resolution, double* cheb_lo) {
+  static std::mutex mu;
+  static std::map<std::array<double, 6>, std::pair<int, double>> memo;
+  const std::array<double, 6> key = {lmax, (double)guess, (double)refine,
+                                     (double)refine_sweeps, (double)kmax, resolution};
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = memo.find(key);
+    if (it != memo.end()) {
+      if (cheb_lo) *cheb_lo = it->second.second;
+      return it->second.first;
+    }
+  }
+  double lo = 1.0;
+  const int m = auto_sweeps_uncached(lmax, guess, refine, refine_sweeps, kmax, resolution, &lo);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    memo[key] = {m, lo};
+  }
+  if (cheb_lo) *cheb_lo = lo;
+  return m;
+}
+
+static int auto_sweeps_uncached(double lmax, int guess, int refine, int refine_sweeps,
+                                int kmax, double resolution, double* cheb_lo) {
+  if (cheb_lo) *cheb_lo = 1.0;
   const int k0 = stable_sweeps(lmax, guess, refine, refine_sweeps, kmax);
   if (k0 < 0) return -1;
   const double tail = std::pow(cheb_bound(lmax, refine_sweeps), refine);
   const double stiff_max = 1.0 - 1.0 / kStiffSteps;
   const double r = resolution > kFineResolution ? kFineResolution / resolution : 1.0;
   const double accuracy = kAccuracy * r * r;
+  // Raised interval [lo, lmax] for the first solve: fine time steps without refinement.
+  if (refine == 0 && resolution <= kFineResolution && lmax > 2.0 && lmax <= kMaxRaisedLambda) {
+    // The fewest sweeps m for which some lo on a 0.02 grid of [1, kMaxChebLo] meets every
+    // condition, with the smallest such lo.  Over lo, max|P| on [lo, lmax] falls and |P(1)|
+    // rises, so the closed-form bounds admit one run of grid points; the stiff-mode
+    // amplification falls as lo grows, so its threshold is bisected within that run.
+    const int nlo = (int)std::lround((kMaxChebLo - 1.0) / 0.02);
+    for (int m = 2; m <= kmax; ++m) {
+      auto bounds = [&](double lo) {
+        const double z0 = (lmax + lo) / (lmax - lo);
+        const double bint = 1.0 / cheb_t(m, z0);  // max |P| over [lo, lmax]
+        const double p1 = std::fabs(cheb_t(m, (lmax + lo - 2.0) / (lmax - lo))) * bint;
+        return lmax * bint <= 0.25 && bint <= accuracy && p1 <= kSmoothAccuracy;
+      };
+      // a coarse grid first: its maximum bounds the full grid's from below, so a coarse
+      // failure is a failure
+      auto damped = [&](double lo) {
+        return amplification_grid(lmax, guess, m, 0, 0, 2.0, lo, 256) <= stiff_max &&
+               amplification_grid(lmax, guess, m, 0, 0, 1.0, lo, 256) <= 1.0 + 1e-9 &&
+               step_amplification(lmax, guess, m, 0, 0, 2.0, lo) <= stiff_max &&
+               step_amplification(lmax, guess, m, 0, 0, 1.0, lo) <= 1.0 + 1e-9;
+      };
+      int ja = -1, jb = -1;
+      for (int j = 0; j <= nlo; ++j)
+        if (bounds(1.0 + 0.02 * j)) {
+          if (ja < 0) ja = j;
+          jb = j;
+        }
+      if (ja < 0) continue;
+      int a = ja, b = jb;  // damped at b; not at a
+      if (damped(1.0 + 0.02 * ja)) b = ja;
+      else if (jb == ja || !damped(1.0 + 0.02 * jb)) continue;
+      while (b - a > 1) {
+        const int mid = (a + b) / 2;
+        if (damped(1.0 + 0.02 * mid)) b = mid;
+        else a = mid;
+      }
+      if (cheb_lo) *cheb_lo = 1.0 + 0.02 * b;
+      return m;
+    }
+  }
   for (int k = k0; k <= kmax; ++k)
     if (lmax * cheb_bound(lmax, k) * tail <= 0.25 &&
         (lmax <= 2.0 || cheb_bound(lmax, k) * tail <= accuracy) &&
@@ -177,7 +267,7 @@ int phs_stable_sweeps(double lambda_max, int32_t guess, int32_t refine, int32_t 
 
 int phs_auto_sweeps(double lambda_max, int32_t guess, int32_t refine, int32_t refine_sweeps,
                     int32_t model, const double* params, double dt, int32_t* sweeps,
-                    double* resolution) {
+                    double* resolution, double* lambda_lo) {
   if (!sweeps || !params || (model != 0 && model != 1) || !(dt > 0))
     return phs::fail(PHS_ERR_ARG, "invalid arguments");
   if (guess < 0 || guess > 2 || refine < 0 || refine > 4 ||
@@ -185,7 +275,8 @@ int phs_auto_sweeps(double lambda_max, int32_t guess, int32_t refine, int32_t re
     return phs::fail(PHS_ERR_ARG, "invalid stability query");
   const double s = phs::step_resolution(model, params, dt);
   if (resolution) *resolution = s;
-  *sweeps = phs::auto_sweeps(lambda_max, guess, refine, refine > 0 ? refine_sweeps : 0, 64, s);
+  *sweeps = phs::auto_sweeps(lambda_max, guess, refine, refine > 0 ? refine_sweeps : 0, 64, s,
+                             lambda_lo);
   return *sweeps < 0 ? phs::fail(PHS_ERR_UNSUPPORTED, "no stable sweep count <= 64") : PHS_OK;
 }
 

@@ -33,8 +33,9 @@ int chebyshev_sweeps(double lmin, double lmax, double tol);
 // (exact solve: P = 0, u' = (2/lam - 1) u, the CN factor).  Returns the largest root
 // modulus over a 4097-point grid of [lam_lo, lmax] (default the whole spectrum [1, lmax]);
 // > 1 means the step grows that mode.
+// cheb_lo: lower end of the Chebyshev interval of the solves, [cheb_lo, lmax].
 double step_amplification(double lmax, int guess, int sweeps, int refine, int refine_sweeps,
-                          double lam_lo = 1.0);
+                          double lam_lo = 1.0, double cheb_lo = 1.0);
 // Fewest sweeps (2..kmax) whose step amplification is <= 1, or -1.
 int stable_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax);
 // Automatic count: the fewest stable sweeps m for which
@@ -52,11 +53,29 @@ int stable_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kma
 //      like s^2 for the extrapolated warm start; the constants are those of the 0.4 mm shell at
 //      dt = 0.05 ms (s = 0.025, 6 sweeps), which meets the activation-time and electrogram bars
 //      over 40 000 steps, so steps that coarse or finer keep the first two conditions only.
+// At fine time steps without refinement the first solve may run its Chebyshev semi-iteration
+// on a raised interval [lo, lmax], lo in [1, kMaxChebLo] (*cheb_lo, else 1; lo = 1 + dt c k^2 / 2
+// for a mode of wavenumber k whatever the voxel size, so the cap keeps the modes that the
+// validation below solved to the interval's accuracy solved so everywhere): conditions (1)-(3)
+// then hold with max|P| taken over [lo, lmax] (a closed form), (2) for the polynomial of that
+// interval, and the smoothest mode's solve error |P(1)| (the largest on [1, lo]) must stay
+// <= kSmoothAccuracy.  Raising lo damps the stiff modes with fewer sweeps and leaves more of
+// the smooth modes' warm-start residual, which the extrapolated warm start keeps small: on
+// the 0.4 mm shell (lambda_max 8.5) 5 sweeps on [1.4, 8.5] (|P(1)| 0.14) and [1.5, 8.5]
+// (0.16) kept every activation within 1 step of the plain FP64 form over 20 000 steps and
+// the chain electrogram within rel-L2 2e-4 (6 sweeps on [1, 8.5]: 3e-5).  The fewest such m
+// with the smallest lo on a 0.02 grid is taken; it is never more sweeps than on [1, lmax].
+// The raised interval is limited to lmax <= kMaxRaisedLambda: on the 0.2 mm shell with scar
+// (lmax 31) the rule's 14 sweeps on [1.38, 31] left 212 of 1.3 M activations unmatched and
+// one 2 steps late over 6000 steps, where 17 on [1, 31] match them all.
 constexpr double kStiffSteps = 40.0;
 constexpr double kAccuracy = 0.03;
 constexpr double kFineResolution = 0.025;
+constexpr double kSmoothAccuracy = 0.16;
+constexpr double kMaxChebLo = 1.5;
+constexpr double kMaxRaisedLambda = 10.0;
 int auto_sweeps(double lmax, int guess, int refine, int refine_sweeps, int kmax,
-                double resolution = 0.0);
+                double resolution = 0.0, double* cheb_lo = nullptr);
 // dt times the steepest upstroke rate of the cell model at rest-to-peak (h = 1 for
 // Mitchell-Schaeffer, h = 0 for Aliev-Panfilov): MS (4/27) / tau_in, AP max_u k u (u - a) (1 - u).
 // params as in phs_sim_desc.

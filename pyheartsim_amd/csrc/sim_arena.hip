@@ -703,9 +703,10 @@ int phs_arena_add(phs_arena* a, const phs_sim_desc* d, const phs_outputs* out, i
                     c.sweeps, h.lmax, c.guess, c.refine, need);
       return undo(PHS_ERR_ARG, msg);
     }
+    // a fixed count runs on [1, lmax]; the automatic one may raise the interval's lower end
     h.need_sweeps = c.sweeps > 0 ? c.sweeps
                                  : phs::auto_sweeps(h.lmax, c.guess, c.refine, rs, kMaxSweeps,
-                                                    resolution);
+                                                    resolution, &h.lmin);
   }
   std::vector<phs::ChebCoef> coef(kMaxSweeps);
   SlotRec sr;

@@ -253,14 +253,15 @@ def stable_sweeps(lambda_max: float, guess: int = 2, refine: int = 0, refine_swe
 def auto_sweeps(lambda_max: float, prep: "Prepared", guess: int = 2, refine: int = 0,
                 refine_sweeps: int = 8):
     """(the arena's automatic sweep count for simulation `prep`, its time resolution s = dt
-    times the cell model's steepest upstroke rate) (phs_auto_sweeps)."""
+    times the cell model's steepest upstroke rate, the lower end of the Chebyshev interval
+    [lo, lambda_max] of its first solve) (phs_auto_sweeps)."""
     lib = _lib.load()
     p = np.ascontiguousarray(prep.params, dtype=np.float64)
-    m, s = C.c_int32(), C.c_double()
+    m, s, lo = C.c_int32(), C.c_double(), C.c_double()
     _lib.check(lib.phs_auto_sweeps(float(lambda_max), guess, refine, refine_sweeps, prep.model,
-                                   _lib.ptr(p), float(prep.dt), C.byref(m), C.byref(s)),
-               "phs_auto_sweeps")
-    return m.value, s.value
+                                   _lib.ptr(p), float(prep.dt), C.byref(m), C.byref(s),
+                                   C.byref(lo)), "phs_auto_sweeps")
+    return m.value, s.value, lo.value
 
 
 def step_amplification(lambda_max: float, sweeps: int, guess: int = 2, refine: int = 0,

@@ -1,5 +1,5 @@
 """Time-step stability of the inexact CN step (host-only C ABI: phs_step_amplification,
-phs_stable_sweeps).
+phs_stable_sweeps, phs_auto_sweeps).
 
 Pins:
   * the per-mode growth factor equals the spectral radius of the mode map built here
@@ -9,12 +9,19 @@ Pins:
   * a 1D chain with zero-flux ends, stepped on the CPU with the same Chebyshev recurrence,
     grows without bound two sweep counts below the stability limit and stays bounded at
     the limit (a plain-matrix restatement, no library code);
-  * the counts measured on the 0.2 mm shell (lambda_max = 31): 14 sweeps is the stability
-    boundary with warm start, 16 the automatic count; 6 for the 0.4 mm shell (8.5);
-  * the automatic count is the fewest stable sweeps that meet the accuracy bound and make
-    every stiff mode (lam >= 2) shrink by e within 40 steps, recomputed from the mode
-    matrices; with extrapolated warm start (guess 2) at lambda_max 31 that is 17 (16, the
-    stability boundary, gave activation mismatches on the 0.2 mm shell; 17 met LAT <= 1).
+  * the stability boundaries on [1, lambda_max] measured on the shells: 14 sweeps at
+    lambda_max 31 (0.2 mm) and 5 at 8.5 (0.4 mm) with the previous-step warm start;
+  * the automatic rule recomputed here from the mode matrices and closed-form Chebyshev
+    bounds: at fine time steps the fewest sweeps m for which some lower end lo of the
+    interval [lo, lambda_max] (a 0.02 grid of [1, 1.5]) keeps the step stable, shrinks every
+    stiff mode (lam >= 2) by e within 40 steps, bounds the error on the interval by
+    1/(4 lambda_max) and 0.03, and the smoothest mode's error |P(1)| by 0.16; the smallest
+    such lo, for lambda_max <= 10.  With the extrapolated warm start that is 5 sweeps on
+    [1.42, 8.5] for the 0.4 mm shell (validated over 20 000 steps against the plain FP64
+    form: LAT within 1 step, no mismatched activation, chain electrogram rel-L2 <= 2e-4).
+    Larger spectra (0.2 mm: 31, where 14 sweeps on [1.38, 31] left activations unmatched)
+    and coarse time steps keep the interval [1, lambda_max]: 17 sweeps at 31, and the
+    accuracy bound 0.03 (0.025 / s)^2 at coarse steps.
 """
 import numpy as np
 import pytest
@@ -44,14 +51,26 @@ def cheb_solve(apply_a, b, x0, lmax, m):
     return x
 
 
-def err_poly(lam, lmax, m):
-    return cheb_solve(lambda v: lam * v, 0.0 * lam, np.ones_like(lam), lmax, m)
+def err_poly(lam, lmax, m, lo=1.0):
+    """Error polynomial of m Chebyshev sweeps on [lo, lmax] at lam (the recurrence applied to
+    the scalar problem lam x = 0 from x = 1)."""
+    c = cheb_coefs(lo, lmax, m)
+    xp = np.ones_like(lam)
+    x = xp - c[0][2] * lam * xp
+    for k in range(1, m):
+        a, bb, g = c[k]
+        x, xp = a * x + bb * xp - g * lam * x, x
+    return x
 
 
-def mode_radius(lam, lmax, m, guess, refine=0, rs=8):
-    P = err_poly(np.array([lam]), lmax, m)[0]
+def mode_radius(lam, lmax, m, guess, refine=0, rs=8, lo=1.0):
+    P = err_poly(np.array([lam]), lmax, m, lo)[0]
     for _ in range(refine):
         P *= err_poly(np.array([lam]), lmax, rs)[0]
+    return radius_of(lam, P, guess)
+
+
+def radius_of(lam, P, guess):
     c = 2.0 * (1.0 - lam) / lam          # d* = c u
     if guess == 0:
         M = np.array([[1.0 + (1.0 - P) * c]])
@@ -87,10 +106,11 @@ def test_exact_solve_reduces_to_cn_factor():
 
 
 def test_counts_measured_on_shells():
-    assert runtime.stable_sweeps(31.0, 1) == (14, 16)
-    assert runtime.stable_sweeps(8.5, 1) == (5, 6)
-    assert runtime.stable_sweeps(31.0, 2) == (16, 17)
-    assert runtime.stable_sweeps(8.5, 2) == (6, 6)
+    assert runtime.stable_sweeps(31.0, 1)[0] == 14
+    assert runtime.stable_sweeps(8.5, 1)[0] == 5
+    assert runtime.stable_sweeps(31.0, 2)[0] == 16
+    assert runtime.stable_sweeps(8.5, 2) == (6, 5)
+    assert runtime.stable_sweeps(31.0, 2)[1] == 17
     assert runtime.step_amplification(31.0, 13, 1) > 1.005
     assert runtime.step_amplification(31.0, 14, 1) <= 1.0 + 1e-9
 
@@ -127,13 +147,21 @@ def test_chain_diverges_below_limit_and_not_at_it():
     assert _chain_run(stable - 2, lmax) > 1e3 * start
 
 
-def _stiff_radius(lmax, m, guess):
-    return max(mode_radius(l, lmax, m, guess) for l in np.linspace(2.0, lmax, 4097))
+def _stiff_radius(lmax, m, guess, lo=1.0, start=2.0, n=4097):
+    lams = np.linspace(start, lmax, n)
+    P = err_poly(lams, lmax, m, lo)
+    return max(radius_of(l, p, guess) for l, p in zip(lams, P))
 
 
-def _bound(lmax, m):
-    q = (np.sqrt(lmax) - 1.0) / (np.sqrt(lmax) + 1.0)
-    return 2.0 * q ** m / (1.0 + q ** (2 * m))
+def _bound(lmax, m, lo=1.0):
+    """max |P| over [lo, lmax] = 1 / T_m((lmax + lo) / (lmax - lo))."""
+    return 1.0 / np.cosh(m * np.arccosh((lmax + lo) / (lmax - lo)))
+
+
+def _p1(lmax, m, lo):
+    """|P(1)|: T_m at the image of lam = 1 (outside the interval for lo > 1)."""
+    z = (lmax + lo - 2.0) / (lmax - lo)
+    return np.cosh(m * np.arccosh(max(z, 1.0))) * _bound(lmax, m, lo)
 
 
 def _accurate(lmax, m, resolution=0.0):
@@ -141,37 +169,79 @@ def _accurate(lmax, m, resolution=0.0):
     return lmax * _bound(lmax, m) <= 0.25 and (lmax <= 2.0 or _bound(lmax, m) <= target)
 
 
+def _fine_ok(lmax, m, lo, guess):
+    """Every condition of the rule at fine time steps for m sweeps on [lo, lmax]."""
+    b = _bound(lmax, m, lo)
+    if not (lmax * b <= 0.25 and b <= 0.03 and _p1(lmax, m, lo) <= 0.16):
+        return False
+    # a coarse grid (a subset of the fine one) first: failing there is failing
+    for n in (257, 4097):
+        if _stiff_radius(lmax, m, guess, lo, 2.0, n) > 1.0 - 1.0 / 40.0:
+            return False
+        if _stiff_radius(lmax, m, guess, lo, 1.0, n) > 1.0 + 1e-9:
+            return False
+    return True
+
+
+LO_GRID = [1.0 + 0.02 * j for j in range(26)]
+
+
 @pytest.mark.parametrize("lmax,guess", [(3.0, 2), (8.5, 1), (8.5, 2), (16.0, 2), (31.0, 1),
                                         (31.0, 2), (60.0, 1)])
 def test_automatic_count_rule(lmax, guess):
-    stable, auto = runtime.stable_sweeps(lmax, guess)
+    """The library's (count, interval) at fine time steps meets the rule, no smaller count
+    meets it with any interval of the grid, and no lower interval end meets it at that count."""
+    from tests.helpers import small_shell
+    w = small_shell(dt=0.05, t_final=0.1)
+    P = w["P_2d"]
+    prep = runtime.prepare(P.shape[0], P, w["geometry"], w["sim"], w["arr"])
+    m, s, lo = runtime.auto_sweeps(lmax, prep, guess=guess)
+    assert s <= 0.025
+    assert runtime.stable_sweeps(lmax, guess)[1] == m
+    stable, _ = runtime.stable_sweeps(lmax, guess)
+    old = next(k for k in range(stable, 65) if _accurate(lmax, k)
+               and _stiff_radius(lmax, k, guess) <= 1.0 - 1.0 / 40.0)
+    if lmax > 10.0:           # the interval stays [1, lambda_max]: the rule on it alone
+        assert lo == 1.0 and m == old
+        return
+    j = round((lo - 1.0) / 0.02)
+    assert abs(lo - LO_GRID[j]) <= 1e-12
+    assert _fine_ok(lmax, m, lo, guess)
+    assert not any(_fine_ok(lmax, m, x, guess) for x in LO_GRID[:j])
+    assert not any(_fine_ok(lmax, k, x, guess) for k in range(2, m) for x in LO_GRID)
+    assert m <= old           # never more sweeps than the rule on [1, lambda_max] alone
 
-    def ok(m):
-        return (m >= stable and _accurate(lmax, m)
-                and _stiff_radius(lmax, m, guess) <= 1.0 - 1.0 / 40.0)
 
-    assert ok(auto)
-    assert not any(ok(m) for m in range(stable, auto))
-
-
-@pytest.mark.parametrize("dt,expect", [(0.05, 6), (0.5, 16)])
-def test_automatic_count_follows_time_resolution(dt, expect):
-    """The 0.4 mm shell's spectrum at dt = 0.05 ms keeps the fine-step count (6); the
-    reference's patient-mesh step dt = 0.5 ms (s = 0.25, ten times coarser) needs the
-    Chebyshev bound below 0.03 / 100: 16 sweeps at lambda_max = 13 (1 mm voxels)."""
+@pytest.mark.parametrize("dt,expect,expect_lo", [(0.05, 5, 1.42), (0.5, 16, 1.0)])
+def test_automatic_count_follows_time_resolution(dt, expect, expect_lo):
+    """The 0.4 mm shell's spectrum at dt = 0.05 ms (s = 0.025) takes the fine-step rule: 5
+    sweeps on [1.42, 8.5].  The reference's patient-mesh step dt = 0.5 ms (s = 0.25, ten
+    times coarser) keeps [1, lambda_max] and needs the Chebyshev bound below 0.03 / 100:
+    16 sweeps at lambda_max = 13 (1 mm voxels)."""
     from tests.helpers import small_shell
     w = small_shell(dt=dt, t_final=2 * dt)
     P = w["P_2d"]
     prep = runtime.prepare(P.shape[0], P, w["geometry"], w["sim"], w["arr"])
     lmax = 8.5 if dt == 0.05 else 13.0
-    m, s = runtime.auto_sweeps(lmax, prep, guess=2)
+    m, s, lo = runtime.auto_sweeps(lmax, prep, guess=2)
     # Mitchell-Schaeffer: s = dt (4/27) / tau_in
     assert s == pytest.approx(dt * (4.0 / 27.0) / 0.3, rel=1e-12)
-    assert m == expect
-    stable, _ = runtime.stable_sweeps(lmax, 2)
+    assert m == expect and lo == pytest.approx(expect_lo, abs=1e-12)
+    if dt > 0.05:
+        stable, _ = runtime.stable_sweeps(lmax, 2)
 
-    def ok(k):
-        return (k >= stable and _accurate(lmax, k, s)
-                and _stiff_radius(lmax, k, 2) <= 1.0 - 1.0 / 40.0)
+        def ok(k):
+            return (k >= stable and _accurate(lmax, k, s)
+                    and _stiff_radius(lmax, k, 2) <= 1.0 - 1.0 / 40.0)
 
-    assert ok(m) and not any(ok(k) for k in range(stable, m))
+        assert ok(m) and not any(ok(k) for k in range(stable, m))
+
+
+def test_refinement_keeps_the_whole_interval():
+    """With FP64-residual refinement the interval stays [1, lambda_max]."""
+    from tests.helpers import small_shell
+    w = small_shell(dt=0.05, t_final=0.1)
+    P = w["P_2d"]
+    prep = runtime.prepare(P.shape[0], P, w["geometry"], w["sim"], w["arr"])
+    m, s, lo = runtime.auto_sweeps(24.5, prep, guess=2, refine=3, refine_sweeps=12)
+    assert lo == 1.0 and m >= 2
