@@ -75,14 +75,19 @@ def _profile_record(path, match, key):
     return None
 
 
-def profiled_traffic(cnt):
-    """HBM bytes (read + write) of one middle-sweep launch, when a committed summary was
-    measured on an arena of the same voxels and bricks."""
+def profiled_traffic(cnt, kind="mid_sweep"):
+    """HBM bytes (read + write) of one launch of kernel `kind` (rhs, first_sweep, mid_sweep,
+    last_sweep), when a committed summary was measured on an arena of the same voxels and
+    bricks."""
+    match = {"voxels": cnt["voxels"], "bricks": cnt["bricks"], "record_bytes": BRICK_RECORD_BYTES}
     for path in PROFILE_TRAFFIC:
-        r = _profile_record(path, {"voxels": cnt["voxels"], "bricks": cnt["bricks"],
-                                   "record_bytes": BRICK_RECORD_BYTES}, "sweep_bytes_per_launch")
-        if r is not None:
-            return r
+        per = _profile_record(path, match, "kernels")
+        if per is not None and per.get(kind) is not None:
+            return per[kind]
+        if kind == "mid_sweep":
+            r = _profile_record(path, match, "sweep_bytes_per_launch")
+            if r is not None:
+                return r
     return None
 
 
@@ -141,18 +146,35 @@ def lat_error(a, b, n_steps=None):
                 compared=int(d.size))
 
 
+def sweep_bytes(k, m, guess=2):
+    """Algorithmic bytes per voxel of Chebyshev sweep k (1..m-1) of an m-sweep solve: it reads
+    d_k (r for the first sweep, k = 1), r (k >= 2) and d_k-1 (k >= 3; at k = 2 it is r / theta);
+    a middle sweep writes d_k+1; the last one (k = m - 1) also reads u*, h, the flag byte and
+    the kept corrections (d_1 with guess >= 1, the warm start with guess 2) and writes u*, h
+    and those corrections (the next step's reaction included)."""
+    rd = 4 + (4 if k >= 2 else 0) + (4 if k >= 3 else 0)
+    if k < m - 1:
+        return rd + 4
+    kept = (4 if guess >= 1 else 0) + (4 if guess == 2 else 0)
+    return rd + 4 + 4 + 1 + kept + 4 + 4 + kept
+
+
 def step_bytes(cnt, sweeps, guess=2, samples_per_step=0.0):
     """Algorithmic HBM bytes of one arena step: each array element read or written once by
-    each kernel, the brick record once per kernel and brick (it carries the scar and
-    temporary-block lanes, so no other kernel than the last reads a flag byte), and the
-    sample row.  Kernels: right-hand side (u*, warm start in; r out), first sweep (r in; d
-    out), sweeps 2..m-1 (d_k, d_k-1, r in; d_k+1 out), last sweep (d_k, d_k-1, r, u*, h,
-    flags, d_1, warm start in; u*, h, d_1, warm start out, next step's reaction included)."""
+    each kernel (sweep_bytes), the brick record once per kernel and brick (it carries the scar
+    and temporary-block lanes, so no other kernel than the last reads a flag byte), and the
+    sample row.  Kernels: right-hand side (u*, warm start in; r out), the first sweep, the
+    middle sweeps (their average under "mid") and the last sweep."""
     V, B = cnt["voxels"], cnt["bricks"]
     rec = B * BRICK_RECORD_BYTES
-    last = 50 if guess == 2 else (42 if guess == 1 else 34)
-    per = dict(rhs=12 * V + rec, first=8 * V + rec, mid=16 * V + rec, last=last * V + rec)
-    total = per["rhs"] + per["first"] + (sweeps - 3) * per["mid"] + per["last"] + \
+    m = max(int(sweeps), 2)
+    mids = [sweep_bytes(k, m, guess) for k in range(2, m - 1)]
+    per = dict(rhs=12 * V + rec, first=sweep_bytes(1, m, guess) * V + rec,
+               mid=(sum(mids) / len(mids)) * V + rec if mids else None,
+               last=sweep_bytes(m - 1, m, guess) * V + rec)
+    if m == 2:  # the first sweep is the last
+        per["first"] = per["last"]
+    total = per["rhs"] + sum(sweep_bytes(k, m, guess) * V + rec for k in range(1, m)) + \
         4 * V * samples_per_step
     return total, per
 
@@ -162,7 +184,9 @@ def step_roofline(cnt, timing, sweeps, ms_per_step, guess=2, isolated=None):
     launch durations of one step measured in place inside a replayed graph of the timed region
     (HIP events between the launches, on the arena's stream).  The dominant kernel is the kind
     with the largest total time per step (the intermediate Chebyshev sweep, launched m - 3
-    times); achieved = its algorithmic bytes per launch / its average launch duration."""
+    times, or the last sweep when that takes longer); achieved = its algorithmic bytes per
+    launch / its average launch duration; traffic = its HBM bytes per launch from the committed
+    counter summary of the same arena (None when there is none)."""
     total, per = step_bytes(cnt, sweeps, guess)
     key = {"rhs": "rhs", "first_sweep": "first", "mid_sweep": "mid", "last_sweep": "last"}
     kinds = {}
@@ -177,13 +201,15 @@ def step_roofline(cnt, timing, sweeps, ms_per_step, guess=2, isolated=None):
         table[kind] = dict(launches=n, avg_ms=avg, bytes_per_launch=b,
                            gbs=(b / (avg * 1e-3) / 1e9) if b else None,
                            frac=(b / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS) if b else None)
+    for kind in table:
+        table[kind]["traffic"] = profiled_traffic(cnt, kind)
     dom = max(kinds, key=lambda k: kinds[k][1])
     d = table[dom]
     in_step = sum(ms for _, ms in timing)
     return {
         "bound": "hbm", "kernel": f"{dom} (k_sweep / k_rhs of the bricked FP32 arena)",
         "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"],
-        "traffic": profiled_traffic(cnt) if dom == "mid_sweep" else None,
+        "traffic": d["traffic"],
         "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_ms"],
         "launches_per_step": d["launches"],
         "share_of_step": kinds[dom][1] / in_step if in_step > 0 else None,
