@@ -817,7 +817,7 @@ def parser():
     ap.add_argument("--graph-steps", type=int, default=100,
                     help="steps per replayed hipGraph (and per electrogram contraction)")
     ap.add_argument("--ring", type=int, default=16, help="sample ring rows per simulation")
-    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--qps", type=float, default=4.0)
     ap.add_argument("--electrodes", type=int, default=64, help="egm workload")
     ap.add_argument("--samples", type=int, default=2000, help="egm workload: time samples")
