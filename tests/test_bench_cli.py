@@ -50,3 +50,30 @@ def test_profiled_traffic_of_the_electrogram():
         algorithmic = 4.0 * t * n + 4.0 * e * n + 4.0 * t * e
         assert rec is not None and algorithmic < rec < 1.3 * algorithmic
     assert bench.egm_traffic(199446, 256, 2000) is None
+
+
+def test_sweep_bytes_follow_the_kernels():
+    """Bytes per voxel of each Chebyshev sweep: the first reads r and writes d; k = 2 also reads
+    r (d_1 = r / theta is not read); k >= 3 also d_k-1; the last reads u*, h, the flag byte
+    and the kept corrections and writes u*, h and the corrections."""
+    assert [bench.sweep_bytes(k, 6) for k in range(1, 6)] == [8, 12, 16, 16, 45]
+    assert bench.sweep_bytes(4, 5, guess=1) == 37 and bench.sweep_bytes(4, 5, guess=0) == 29
+    assert bench.sweep_bytes(2, 3) == 41          # last sweep at k = 2: no d_k-1
+    total, per = bench.step_bytes({"voxels": 1, "bricks": 0}, 5)
+    assert total == 12 + 8 + 12 + 16 + 45 and per["mid"] == 14.0
+
+
+def test_algorithmic_bytes_match_the_measured_traffic():
+    """The committed counter summary of the batched arena moves each kernel's algorithmic
+    bytes within 5 % (the byte accounting is checked against the hardware, not restated)."""
+    import json
+    recs = [r for r in _traffic_lines() if "kernels" in r]
+    assert recs
+    for rec in recs:
+        cnt = {"voxels": rec["voxels"], "bricks": rec["bricks"]}
+        _, per = bench.step_bytes(cnt, rec["sweeps"])
+        for kind, key in (("rhs", "rhs"), ("first_sweep", "first"), ("mid_sweep", "mid"),
+                          ("last_sweep", "last")):
+            measured = bench.profiled_traffic(cnt, kind)
+            assert measured == rec["kernels"][kind]
+            assert abs(measured / per[key] - 1.0) <= 0.05, (kind, measured, per[key])
