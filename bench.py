@@ -224,6 +224,25 @@ def step_roofline(cnt, timing, sweeps, ms_per_step, guess=2, isolated=None):
 
 
 # ------------------------------------------------------------------------- workload builders
+def copy_probe(torch, mib=1024, reps=5):
+    """Device-to-device copy rate (read + write bytes per second, GB/s) of a `mib` MiB buffer,
+    timed with events on torch's stream: a practical HBM ceiling beside the nominal peak."""
+    n = mib * (1 << 20) // 4
+    src = torch.empty(n, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del src, dst
+    return dict(gbs=2.0 * n * 4 / (ms * 1e-3) / 1e9, mib=mib,
+                method="torch copy_ of a device buffer to another, HIP events, read + write bytes")
+
+
 def _batched_member(kw):
     """One member of the batched workload, built on the host (runs in a worker process)."""
     from pyheartsim_amd import configs, runtime
@@ -265,6 +284,7 @@ class Chain:
         self.members = members
         self.ring = ring
         n_vox = sum(m["prep"].n_voxel for m in members)
+        self.solver = solver
         self.arena = runtime.Arena(n_vox, len(members), solver)
         self.slots = [self.arena.add(m["prep"], samples=egm, ring=ring if egm else 0)
                       for m in members]
@@ -405,6 +425,29 @@ def cpu_baseline(kw, u0, h0, step0, budget_s):
                 maxiter_hits=int(fails), seconds=dt)
 
 
+def sample_error(torch, runtime, solver, prep, plain_su):
+    """Vm error of the chosen form over every 1 kHz sample of the whole run: the simulation
+    re-run alone (bitwise what it computes packed) with all its samples kept, against the
+    plain FP64 form's samples (caller voxel order).  max and p99 of |dVm| over all
+    (sample, voxel) pairs, the p99 on a seeded subsample of 4 M pairs."""
+    a = runtime.Arena(prep.n_voxel, 1, solver)
+    s = a.add(prep, samples=True)
+    while a.steps_done(s) < prep.n_steps:
+        a.step(100, graph=True)
+    su = a.outputs(s)[0]
+    d = (su - plain_su).abs()
+    n = d.numel()
+    g = torch.Generator(device="cuda").manual_seed(5)
+    pick = torch.randint(0, n, (min(n, 4_000_000),), device="cuda", generator=g)
+    p99 = float(torch.quantile(d.reshape(-1)[pick], 0.99).item())
+    out = dict(max=float(d.max().item()), p99=p99,
+               rel_max=float(d.max().item() / plain_su.abs().max().item()),
+               samples=int(su.shape[0]))
+    a.close()
+    del su, d
+    return out
+
+
 def validate_member(torch, runtime, chain, i, lf_cls, cpu_state_step=None):
     """Plain FP64 run of member i over its whole duration (timed), compared with the chosen
     form already finished in `chain`: LAT error and the electrogram chain rel-L2.  Returns
@@ -436,9 +479,10 @@ def validate_member(torch, runtime, chain, i, lf_cls, cpu_state_step=None):
     dvm = float((chain.arena.state(chain.slots[i])[0].double() - plain.state()[0]).abs().max()
                 .item())
     lf.close()
+    samples = sample_error(torch, runtime, chain.solver, prep, plain.su)
     rec = dict(member=i, n_voxel=prep.n_voxel, steps=prep.n_steps, n_scar=m.get("n_scar"),
                scar_voxels=m.get("scar_voxels"), cycle_length_ms=m.get("cycle_length"),
-               lat_err_steps=le, egm_rel_l2=rel, max_abs_dVm_final=dvm,
+               lat_err_steps=le, egm_rel_l2=rel, max_abs_dVm_final=dvm, dVm_samples=samples,
                plain_voxel_steps_per_s=rate, plain_sweeps=plain.info()[0])
     del plain
     return rec, rate, state
@@ -553,7 +597,10 @@ def run_batched(args, torch, runtime, pre):
     roof = step_roofline(cnt, timing, sweeps, ms_step, args.guess, prof)
     roof["sustained_step_hbm_frac"] = (roof["step_algorithmic_bytes"] /
                                        (sustained["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
-    log(f"kernels: {prof}")
+    cp = copy_probe(torch)
+    roof["copy_probe"] = cp
+    roof["frac_of_copy"] = roof["achieved"] / cp["gbs"] if roof["achieved"] else None
+    log(f"kernels: {prof}; copy probe {cp['gbs']:.0f} GB/s")
     # validation against the plain FP64 form: the largest member and the most scarred one
     big = int(np.argmax(ns))
     scarred = max((i for i in range(len(members)) if i != big),
