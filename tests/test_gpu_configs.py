@@ -199,6 +199,8 @@ def _run_chain(torch, members, chunk=100, ring=16):
 def test_batched_members_packed_equal_alone(cuda):
     torch = cuda
     from pyheartsim_amd import runtime
+    # 24 497 bricks packed (the record prefetch is on from 16 384 bricks), 7 938 / 5 266 /
+    # 11 293 alone (off): the bitwise comparison also covers the prefetch
     members = _batched_members([0, 2, 49], t_final=200.0)
     ns = [m["prep"].n_voxel for m in members]
     print(f"batched members N={ns} scar blobs={[m['n_scar'] for m in members]}", flush=True)
