@@ -422,7 +422,58 @@ def cpu_baseline(kw, u0, h0, step0, budget_s):
                 cpu_model=cpu_model(), os_cpu_count=os.cpu_count(),
                 omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
                 build="gcc -O3 -march=x86-64-v2 -fopenmp -ffp-contract=off",
-                maxiter_hits=int(fails), seconds=dt)
+                maxiter_hits=int(fails), seconds=dt,
+                tight=cpu_tight(w, P, u0, h0, step0, threads, max(budget_s / 5, 0.5)),
+                egm=cpu_egm(w, cs.u, max(budget_s / 5, 0.5)))
+
+
+def cpu_tight(w, P, u0, h0, step0, threads, budget_s):
+    """The same restatement with the CN solve converged tightly (BiCGSTAB rtol 1e-12,
+    atol 1e-14), the accuracy the device forms are held to, from the same state."""
+    from oracle import cpu_oracle
+    cs = cpu_oracle.CpuSim(P, w["geometry"], w["sim"], w["arr"], rtol=1e-12, atol=1e-14)
+    cs.u[:] = u0
+    cs.h[:] = h0
+    cs.step = step0
+    chunk, steps, fails = 5, 0, 0
+    t0 = time.perf_counter()
+    while True:
+        fails += cs.run(chunk, threads, rebuild=True)
+        steps += chunk
+        dt = time.perf_counter() - t0
+        if dt >= budget_s or steps >= 5000:
+            break
+    return dict(value=cs.n * steps / dt, unit="voxel-steps/s", steps=steps, seconds=dt,
+                maxiter_hits=int(fails),
+                sample="BiCGSTAB(atol 1e-14, rtol 1e-12, maxiter 1000), same member and state")
+
+
+def cpu_egm(w, u, budget_s):
+    """The FP64 NumPy restatement of the electrogram (legacy/compute_unipolar_electrogram_cpu.py,
+    unipolar_electrogram_equation_parts.py) on the member's 64 electrodes, timed over calls of
+    32 sample rows (copies of the state u) for ~budget_s seconds (at least one call, whose
+    distances and lead-field terms are formed anew as the reference's function does)."""
+    from oracle import egm_oracle
+    g = w["geometry"]
+    el = np.asarray(w["electrodes"], np.float64)
+    n = g["voxel"].shape[0]
+    D = np.zeros((9, n))
+    D[0] = D[4] = D[8] = 1.0
+    c = np.asarray(w["hm"]["c_voxel"], np.float64)
+    rows = np.tile(np.asarray(u, np.float64), (32, 1))
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        egm_oracle.electrogram_raw(el, g["voxel"], D, c, rows, g["Delta"], g["neighbor_id_2d"])
+        done += rows.shape[0]
+        dt = time.perf_counter() - t0
+        if dt >= budget_s:
+            break
+    E = el.shape[0]
+    return dict(value=done / dt, unit="electrogram rows/s", electrodes=E, n_voxel=n,
+                useful_gflops=2.0 * E * 3 * n * done / dt / 1e9, rows=done, seconds=dt,
+                sample="NumPy FP64 (BLAS threads of the job), calls of 32 rows, distances, "
+                       "lead-field terms and gradients included")
 
 
 def sample_error(torch, runtime, solver, prep, plain_su):

@@ -1,7 +1,8 @@
 """The driver's bench command, in a reduced form: `--steps 20 --warmup 5` on the batched
 workload (4 members instead of 64, no shell legs, no served stream, a 1 s CPU sample).  The
 result must be one JSON-serialisable record with the contract's keys, a roofline and a CPU
-baseline, and its validation members within the LAT and electrogram bars."""
+baseline (with its tight-tolerance and electrogram figures), and its validation members
+within the LAT and electrogram bars."""
 import json
 
 import pytest
@@ -31,6 +32,8 @@ def test_bench_driver_command_reduced(cuda):
     cpu = line["cpu_baseline"]
     assert cpu["value"] > 0 and cpu["cores"] >= 1 and cpu["kind"] == "port"
     assert cpu["cpu_model"]
+    assert cpu["tight"]["value"] > 0 and cpu["tight"]["steps"] > 0
+    assert cpu["egm"]["value"] > 0 and cpu["egm"]["electrodes"] == 64
     for v in line["validate"]:
         assert v["lat_err_steps"]["max"] <= 1
         assert v["lat_err_steps"]["activation_mismatch"] == 0
