@@ -11,7 +11,12 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libphs.so"
 
+PHS_ERR_ARG = -1
+PHS_ERR_HIP = -2
+PHS_ERR_OOM = -3
 PHS_ERR_UNSUPPORTED = -4
+PHS_ERR_CAPACITY = -5
+PHS_ERR_STATE = -6
 
 PHS_VF_SCAR = 0x01
 PHS_VF_TEMP_BLOCK = 0x02
@@ -145,7 +150,12 @@ _lib = None
 
 
 class PhsError(RuntimeError):
-    pass
+    """A failed library call; `code` is its PHS_ERR_* return value (None when the failure
+    was found before any call, e.g. a missing library)."""
+
+    def __init__(self, message: str, code: int | None = None):
+        super().__init__(message)
+        self.code = code
 
 
 def load():
@@ -168,7 +178,7 @@ def load():
 def check(rc: int, what: str = ""):
     if rc != 0:
         msg = load().phs_last_error().decode(errors="replace")
-        raise PhsError(f"{what} failed ({rc}): {msg}")
+        raise PhsError(f"{what} failed ({rc}): {msg}", rc)
 
 
 def ptr(t) -> int:

@@ -111,7 +111,7 @@ def prepare(n_voxel, P_2d, geometry_data, simulation_parameters, arrhythmia_para
         # (fiber_flag 0, simulation/equation_parts.py:19, :32)
         raise _lib.PhsError(f"PHS_ERR_UNSUPPORTED ({_lib.PHS_ERR_UNSUPPORTED}): anisotropic "
                             "diffusion (P_2d columns 0-14 other than D = I) is not supported by "
-                            "the device path")
+                            "the device path", _lib.PHS_ERR_UNSUPPORTED)
     params = P[0, 15:20].copy()
     params_voxel = None
     if np.any(P[:, 15:20] != params[None, :]):

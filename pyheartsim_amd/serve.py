@@ -182,7 +182,7 @@ class Server:
         try:
             slot = self.arena.add(r.prep, samples=True, ring=2 * self.ring)
         except runtime._lib.PhsError as e:
-            if "capacity" in str(e) or "slot" in str(e):
+            if e.code == runtime._lib.PHS_ERR_CAPACITY:  # no free slot, voxels or bricks
                 return None
             raise
         perm = self.arena.slots[slot].perm

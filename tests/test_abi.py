@@ -75,7 +75,9 @@ def test_brick_plan_rejects_inconsistent_neighbours():
     nb[i, 0] = -1                      # a cut link is part of the table's operator
     runtime.brick_plan(g["ijk"], nb)
     nb[i, 0] = nb[i, 1] if nb[i, 1] >= 0 else (i + 1) % len(nb)   # the wrong voxel
-    with pytest.raises(_lib.PhsError, match="neighbour table entry"):
+    with pytest.raises(_lib.PhsError, match="neighbour table entry") as e:
         runtime.brick_plan(g["ijk"], nb)
-    with pytest.raises(_lib.PhsError, match="duplicate"):
+    assert e.value.code == _lib.PHS_ERR_UNSUPPORTED
+    with pytest.raises(_lib.PhsError, match="duplicate") as e:
         runtime.brick_plan(np.concatenate([g["ijk"], g["ijk"][:1]]))
+    assert e.value.code == _lib.PHS_ERR_ARG

@@ -65,3 +65,29 @@ def test_ragged_batch_matches_alone(cuda):
         um, _, lm = mix.outputs(s)
         assert np.array_equal(ua.cpu().numpy(), um.cpu().numpy())
         assert np.array_equal(la.cpu().numpy(), lm.cpu().numpy())
+
+
+def test_full_arena_reports_capacity_and_server_queues(cuda):
+    """An add that does not fit fails with PHS_ERR_CAPACITY (slots, then voxels), and the
+    server keeps such a request queued until a running one finishes."""
+    from pyheartsim_amd import _lib, runtime, serve
+    from tests.helpers import small_shell
+    w = small_shell(t_final=10.0)
+    p = runtime.prepare(w["P_2d"].shape[0], w["P_2d"], w["geometry"], w["sim"], w["arr"])
+    a = runtime.Arena(4 * p.n_voxel, 1, runtime.SolverConfig())
+    a.add(p)
+    with pytest.raises(_lib.PhsError) as e:
+        a.add(p)
+    assert e.value.code == _lib.PHS_ERR_CAPACITY
+    a.close()
+    b = runtime.Arena(p.n_voxel // 2, 2, runtime.SolverConfig())
+    with pytest.raises(_lib.PhsError) as e:
+        b.add(p)
+    assert e.value.code == _lib.PHS_ERR_CAPACITY
+    b.close()
+    reqs = serve.make_requests(3, qps=1000.0, seed=9, n_range=(4e3, 8e3),
+                               duration_range=(20.0, 30.0), electrode_range=(8, 16))
+    srv = serve.Server(max_voxels=20_000, max_sims=1, hbm_budget_bytes=8e9, ring=16,
+                       chunk_steps=50)
+    st = srv.run(reqs)
+    assert st.completed == len(reqs)

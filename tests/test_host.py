@@ -70,8 +70,9 @@ def test_prepare_rejects_anisotropy_keeps_hetero_params():
     w = small_shell()
     P = w["P_2d"].copy()
     P[3, 7] = 0.5
-    with pytest.raises(_lib.PhsError, match="PHS_ERR_UNSUPPORTED"):
+    with pytest.raises(_lib.PhsError, match="PHS_ERR_UNSUPPORTED") as e:
         runtime.prepare(P.shape[0], P, w["geometry"], w["sim"], w["arr"])
+    assert e.value.code == _lib.PHS_ERR_UNSUPPORTED
     P = w["P_2d"].copy()
     P[3, 17] = 0.31
     prep = runtime.prepare(P.shape[0], P, w["geometry"], w["sim"], w["arr"])
