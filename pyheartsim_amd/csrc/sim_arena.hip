@@ -117,6 +117,8 @@ struct phs_arena {
   std::vector<SlotHost> sh;
   RangeAlloc vox, brk;
   int64_t brick_high = 0;
+  void* move_tmp = nullptr;  // scratch for overlapping brick-record moves (phs_arena_compact)
+  size_t move_bytes = 0;
   hipStream_t cap_stream = nullptr;
   // One captured sequence of steps per launch parity at its first step (the kernels' slot
   // copy alternates with the step), so that odd step counts replay without recapture too.
@@ -830,6 +832,70 @@ int phs_arena_remove(phs_arena* a, int32_t slot, void* stream) {
   return PHS_OK;
 }
 
+// Launches cover bricks [0, brick_high), so the ranges that finished simulations leave below
+// the highest resident one are launched for nothing.  A brick record holds voxel indices only
+// (its own start and its neighbours' gather bases, all in its simulation's voxel range), never
+// another brick's index, so a simulation's records can move to any brick range as they are:
+// its voxel data stay where they are.  Compaction slides the simulations down in brick order
+// until they are contiguous from brick 0.
+int phs_arena_compact(phs_arena* a, double max_waste, int32_t* moved, void* stream) {
+  if (!a) return phs::fail(PHS_ERR_ARG, "handle is NULL");
+  if (!(max_waste >= 0.0)) return phs::fail(PHS_ERR_ARG, "max_waste must be >= 0");
+  hipStream_t s = (hipStream_t)stream;
+  if (moved) *moved = 0;
+  std::vector<int> order;
+  int64_t live = 0;
+  for (int i = 0; i < a->cfg.max_sims; ++i)
+    if (a->sh[i].used) {
+      order.push_back(i);
+      live += a->sh[i].n_bricks;
+    }
+  const int64_t launched = (int64_t)grid_for(a) * kWaves * nb_per_wave(a);
+  if (order.empty() || (double)launched <= (1.0 + max_waste) * (double)live) return PHS_OK;
+  std::sort(order.begin(), order.end(),
+            [&](int x, int y) { return a->sh[x].brick_off < a->sh[y].brick_off; });
+  int32_t n_moved = 0;
+  int64_t next = 0;  // bricks [0, next) hold the simulations already placed
+  for (int i : order) {
+    SlotHost& h = a->sh[i];
+    const int64_t n = h.n_bricks;
+    if (h.brick_off > next) {
+      const size_t bytes = sizeof(BrickRec) * (size_t)n;
+      if (next + n > h.brick_off) {  // the ranges overlap: copy through a scratch range
+        if (a->move_bytes < bytes) {
+          hipFree(a->move_tmp);
+          a->move_tmp = nullptr;
+          a->move_bytes = 0;
+          PHS_TRY_ALLOC(a->move_tmp, bytes);
+          a->move_bytes = bytes;
+        }
+        PHS_HIP_CHECK(hipMemcpyAsync(a->move_tmp, a->bricks + h.brick_off, bytes,
+                                     hipMemcpyDeviceToDevice, s));
+        PHS_HIP_CHECK(hipMemcpyAsync(a->bricks + next, a->move_tmp, bytes,
+                                     hipMemcpyDeviceToDevice, s));
+        const int64_t tail = h.brick_off + n - (next + n);  // old bricks above the new range
+        k_invalidate<<<(unsigned)((tail + 255) / 256), 256, 0, s>>>(a->bricks, next + n, tail);
+      } else {
+        PHS_HIP_CHECK(hipMemcpyAsync(a->bricks + next, a->bricks + h.brick_off, bytes,
+                                     hipMemcpyDeviceToDevice, s));
+        k_invalidate<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a->bricks, h.brick_off, n);
+      }
+      PHS_HIP_CHECK(hipGetLastError());
+      // [0, next) is full, so once the old range is free the lowest range that fits n
+      // starts at next
+      a->brk.release(h.brick_off, n);
+      const int64_t at = a->brk.alloc(n);
+      if (at != next) return phs::fail(PHS_ERR_STATE, "brick ranges out of order");
+      h.brick_off = next;
+      ++n_moved;
+    }
+    next = h.brick_off + n;
+  }
+  recompute_high(a);
+  if (moved) *moved = n_moved;
+  return PHS_OK;
+}
+
 int phs_arena_step(phs_arena* a, int32_t n_steps, void* stream) {
   if (!a) return phs::fail(PHS_ERR_ARG, "handle is NULL");
   if (n_steps < 0) return phs::fail(PHS_ERR_ARG, "n_steps < 0");
@@ -1068,6 +1134,7 @@ int phs_arena_destroy(phs_arena* a) {
   hipFree(a->VF);
   hipFree(a->diag_acc);
   hipFree(a->NONFINITE);
+  hipFree(a->move_tmp);
   for (auto& q : a->PV) hipFree(q);
   delete a;
   return PHS_OK;

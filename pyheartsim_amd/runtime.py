@@ -355,6 +355,16 @@ class Arena:
                    "phs_arena_remove")
         self.slots.pop(slot, None)
 
+    def compact(self, max_waste: float = 0.05, stream=None) -> int:
+        """When the launches cover more than (1 + max_waste) times the resident bricks,
+        slide the simulations' brick records down until they are contiguous
+        (phs_arena_compact); returns the number of simulations moved.  Results are
+        unchanged, bit for bit."""
+        moved = C.c_int32()
+        _lib.check(self.lib.phs_arena_compact(self.handle, float(max_waste), C.byref(moved),
+                                              _lib.stream_ptr(stream)), "phs_arena_compact")
+        return moved.value
+
     def step(self, n: int, graph: bool = False, stream=None):
         fn = self.lib.phs_arena_step_graph if graph else self.lib.phs_arena_step
         _lib.check(fn(self.handle, n, _lib.stream_ptr(stream)), "phs_arena_step")

@@ -14,7 +14,9 @@ electrode set) arrive open-loop at Poisson times.  The loop
     full (samples x voxels) history;
   * retires a finished simulation: last partial electrogram chunk, normalisation,
     activation times in caller order, results copied to the host; its arena ranges
-    are reused by the next admission.
+    are reused by the next admission, and the resident simulations' brick records slide
+    down to close the gap (phs_arena_compact), so that the launches shrink with the
+    resident set.
 
 Latency of a request = completion time (its results are on the host) - arrival time.
 """
@@ -256,6 +258,7 @@ class Server:
             while len(inflight) > 2:
                 inflight.pop(0).synchronize()
             keep = []
+            removed = False
             for a in retiring:
                 if a.finished_event.query():
                     t_done = time.perf_counter() - t_start
@@ -269,6 +272,7 @@ class Server:
                                         **({"egm": a.results["egm"], "lat": a.results["lat"]}
                                            if self.keep else {})))
                     self.arena.remove(a.slot)
+                    removed = True
                     a.lf.close()
                     self.in_use -= a.req.footprint_bytes(2 * self.ring)
                     if progress:
@@ -276,6 +280,10 @@ class Server:
                 else:
                     keep.append(a)
             retiring = keep
+            if removed:
+                # close the ranges just freed, so that the launches cover the resident
+                # bricks rather than the arena's high-water mark
+                self.arena.compact()
             if not active and retiring:
                 time.sleep(0.0005)
         torch.cuda.synchronize()

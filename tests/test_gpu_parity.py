@@ -108,6 +108,53 @@ def test_arena_alone_equals_mixed_and_graph(cuda):
     assert np.array_equal(la.cpu().numpy(), lm.cpu().numpy())
 
 
+@pytest.mark.parametrize("removed", [0, 1])
+def test_compaction_moves_records_bitwise(cuda, removed):
+    """Removing a simulation and compacting slides the ones above it down in brick order:
+    with the bottom one removed, the small middle one moves by more than its size and the
+    large top one onto part of its old range; with the small middle one removed, the large
+    top one moves onto part of its old range.  The launches shrink, and the simulations that
+    stay give the same bits as alone, with graph replay across the move."""
+    from pyheartsim_amd import runtime
+    axes = [(9.0, 8.0, 7.0), (6.0, 5.0, 5.0), (10.0, 9.0, 8.0)]
+    ws = [small_shell(t_final=10.0 if i == removed else 30.0, seed=3 + 2 * i, semi_axes=ax,
+                      n_scar=1 if i == 2 else 0) for i, ax in enumerate(axes)]
+    ps = [_prep(w) for w in ws]
+    solver = runtime.SolverConfig()
+    ref = {}
+    for i, p in enumerate(ps):
+        if i == removed:
+            continue
+        one = runtime.Arena(p.n_voxel, 1, solver)
+        s = one.add(p)
+        one.step(p.n_steps)
+        ref[i] = [t.cpu().numpy() for t in one.outputs(s) if t is not None]
+        one.close()
+    ar = runtime.Arena(sum(p.n_voxel for p in ps), 3, solver)
+    slots = [ar.add(p) for p in ps]
+    nb = [ar.slot_info(s)["n_bricks"] for s in slots]
+    assert nb[1] < nb[0] < nb[2]
+    ar.step(ps[removed].n_steps, graph=True)
+    before = ar.counts()
+    ar.remove(slots[removed])
+    assert ar.compact(0.0) >= 1
+    after = ar.counts()
+    assert after["bricks"] == before["bricks"] - nb[removed]
+    assert after["launch_bricks"] < before["launch_bricks"]
+    assert after["launch_bricks"] - after["bricks"] < 8 * 4   # contiguous up to the grid's rounding
+    assert ar.compact(0.0) == 0                                # nothing left to move
+    for _ in range(max(p.n_steps for p in ps) // 50 + 1):
+        ar.step(50, graph=True)
+    for i, s in enumerate(slots):
+        if i == removed:
+            continue
+        got = [t.cpu().numpy() for t in ar.outputs(s) if t is not None]
+        assert len(got) == len(ref[i]) == 2      # samples and activation times
+        for g, r in zip(got, ref[i]):
+            assert np.array_equal(g, r)
+    ar.close()
+
+
 def test_fp32_arena_temp_block_and_scar_vs_oracle(cuda):
     from pyheartsim_amd import runtime
     w = small_shell(t_final=40.0, n_scar=2, temp_block=True)
