@@ -330,12 +330,26 @@ struct Stencil {
   }
 };
 
-// Kernel flavours.  The lean ones run when every simulation in the arena runs the launched
-// sweep count and none has heterogeneous alpha: they drop the per-voxel alpha and the
-// identity-sweep logic (the scalar and vector work they cost is a measurable share of a sweep,
-// which is bound by instruction issue as much as by memory).  FLAV_LEAN also has no
-// temporary-block voxels to test (scar needs no test); FLAV_LEAN_FLAGS keeps that test.
-enum { FLAV_GENERAL = 0, FLAV_LEAN_FLAGS = 1, FLAV_LEAN = 2 };
+// Kernel flavours.  The lean ones run when no simulation in the arena has heterogeneous
+// alpha or per-voxel cell parameters: they drop the per-voxel loads, and each keeps only the
+// tests its arena needs (the scalar and vector work they cost is a measurable share of a
+// sweep, which is bound by instruction issue as much as by memory): the temporary-block test
+// (FLAGS; scar needs no test) and the identity-sweep logic of simulations whose own sweep
+// count is below the launched one (IDENT).
+enum {
+  FLAV_GENERAL = 0,
+  FLAV_LEAN_FLAGS = 1,
+  FLAV_LEAN = 2,
+  FLAV_LEAN_IDENT = 3,
+  FLAV_LEAN_FLAGS_IDENT = 4
+};
+constexpr bool flav_hetero(int f) { return f == FLAV_GENERAL; }
+constexpr bool flav_tb(int f) {
+  return f == FLAV_GENERAL || f == FLAV_LEAN_FLAGS || f == FLAV_LEAN_FLAGS_IDENT;
+}
+constexpr bool flav_ident(int f) {
+  return f == FLAV_GENERAL || f == FLAV_LEAN_IDENT || f == FLAV_LEAN_FLAGS_IDENT;
+}
 
 // Wave-uniform slot values a stencil kernel needs, loaded together.
 // Per-simulation sweep counts: a solve of K launched sweeps carries a simulation whose own
@@ -352,7 +366,7 @@ struct SlotView {
   template <int FLAV = FLAV_GENERAL>
   __device__ __forceinline__ void load(const SlotRec& S, int par, int k, int K = 0,
                                        bool last = true) {
-    if (FLAV != FLAV_GENERAL) {
+    if (!flav_ident(FLAV)) {
       ident = skip = false;
     } else {
       const int nsw = S.nsw;
@@ -384,18 +398,16 @@ struct AlphaIn {
                                        const BrickRec& br) {
     al = 0.0f;
     tblock = false;
-    if (FLAV == FLAV_LEAN) return;
-    if (FLAV == FLAV_GENERAL && V.hetero) al = ld(A.ALPHA, idx);
-    tblock = lane_bit(br.tblock);
+    if (flav_hetero(FLAV) && V.hetero) al = ld(A.ALPHA, idx);
+    if (flav_tb(FLAV)) tblock = lane_bit(br.tblock);
   }
 };
 
 template <typename T, int FLAV = FLAV_GENERAL>
 __device__ __forceinline__ T alpha_at(const SlotView& V, const AlphaIn& in) {
   T a = sizeof(T) == 8 ? (T)V.alpha64 : (T)V.alpha;
-  if (FLAV == FLAV_LEAN) return a;
-  if (FLAV == FLAV_GENERAL && V.hetero) a = (T)in.al;
-  if ((V.cur_sched & PHS_SCHED_TEMP_BLOCK) && in.tblock) a = (T)0;
+  if (flav_hetero(FLAV) && V.hetero) a = (T)in.al;
+  if (flav_tb(FLAV) && (V.cur_sched & PHS_SCHED_TEMP_BLOCK) && in.tblock) a = (T)0;
   return a;
 }
 
@@ -700,7 +712,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
         p2[j] = GUESS == 2 ? ld(A.D2, i) : 0.0f;
 #pragma unroll
         for (int k = 0; k < 5; ++k) pp[j][k] = sf[j].p[k];
-        if (FLAV == FLAV_GENERAL && sf[j].hparams)
+        if (flav_hetero(FLAV) && sf[j].hparams)
 #pragma unroll
           for (int k = 0; k < 5; ++k) pp[j][k] = ld(A.PV[k], i);
       }

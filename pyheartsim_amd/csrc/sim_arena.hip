@@ -215,7 +215,8 @@ int grid_for(const phs_arena* a) {
 
 // Launches of one time step for brick shape (BX, BY, BZ) and NB bricks per wavefront.
 // The lean kernel flavours (one brick per wavefront only) serve arenas whose simulations need
-// neither per-voxel alpha nor identity sweeps.
+// neither per-voxel alpha nor per-voxel cell parameters, with or without the temporary-block
+// test and the identity sweeps of simulations whose own sweep count is below the arena's.
 template <int BX, int BY, int BZ, int NB, int GUESS>
 struct Step {
   template <int MODE, int FLAV>
@@ -242,8 +243,13 @@ struct Step {
 
   static void run(phs_arena* a, hipStream_t s) {
     if constexpr (NB == 1) {
-      if (a->flav == FLAV_LEAN) return run_t<FLAV_LEAN>(a, s);
-      if (a->flav == FLAV_LEAN_FLAGS) return run_t<FLAV_LEAN_FLAGS>(a, s);
+      switch (a->flav) {
+        case FLAV_LEAN: return run_t<FLAV_LEAN>(a, s);
+        case FLAV_LEAN_FLAGS: return run_t<FLAV_LEAN_FLAGS>(a, s);
+        case FLAV_LEAN_IDENT: return run_t<FLAV_LEAN_IDENT>(a, s);
+        case FLAV_LEAN_FLAGS_IDENT: return run_t<FLAV_LEAN_FLAGS_IDENT>(a, s);
+        default: break;
+      }
     }
     run_t<FLAV_GENERAL>(a, s);
   }
@@ -303,8 +309,13 @@ struct Step {
   // the arena launches (of its flavour).
   static hipError_t profile(phs_arena* a, int reps, double* ms, hipStream_t s) {
     if constexpr (NB == 1) {
-      if (a->flav == FLAV_LEAN) return profile_t<FLAV_LEAN>(a, reps, ms, s);
-      if (a->flav == FLAV_LEAN_FLAGS) return profile_t<FLAV_LEAN_FLAGS>(a, reps, ms, s);
+      switch (a->flav) {
+        case FLAV_LEAN: return profile_t<FLAV_LEAN>(a, reps, ms, s);
+        case FLAV_LEAN_FLAGS: return profile_t<FLAV_LEAN_FLAGS>(a, reps, ms, s);
+        case FLAV_LEAN_IDENT: return profile_t<FLAV_LEAN_IDENT>(a, reps, ms, s);
+        case FLAV_LEAN_FLAGS_IDENT: return profile_t<FLAV_LEAN_FLAGS_IDENT>(a, reps, ms, s);
+        default: break;
+      }
     }
     return profile_t<FLAV_GENERAL>(a, reps, ms, s);
   }
@@ -447,13 +458,16 @@ void recompute_high(phs_arena* a) {
     }
   a->brick_high = hi;
   a->sweeps_eff = a->cfg.sweeps > 0 ? a->cfg.sweeps : sw;
-  bool lean = true, flags = false;
+  bool lean = true, flags = false, ident = false;
   for (auto& h : a->sh)
     if (h.used) {
-      if (h.hetero || h.hparams || h.need_sweeps != a->sweeps_eff) lean = false;
+      if (h.hetero || h.hparams) lean = false;
+      ident = ident || h.need_sweeps != a->sweeps_eff;
       flags = flags || h.needs_tb;
     }
-  a->flav = !lean ? FLAV_GENERAL : (flags ? FLAV_LEAN_FLAGS : FLAV_LEAN);
+  a->flav = !lean  ? FLAV_GENERAL
+            : flags ? (ident ? FLAV_LEAN_FLAGS_IDENT : FLAV_LEAN_FLAGS)
+                    : (ident ? FLAV_LEAN_IDENT : FLAV_LEAN);
 }
 
 }  // namespace

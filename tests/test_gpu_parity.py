@@ -86,8 +86,8 @@ def test_fp32_arena_lat_within_one_step(cuda):
 def test_arena_alone_equals_mixed_and_graph(cuda):
     """A simulation gives the same bits alone (lean kernels: no scar, one sweep count) as
     packed with a scarred one (lean kernels with the scar test), admitted mid-run, with
-    graph replay.  The general kernels' bits are checked the same way by
-    test_per_simulation_sweep_counts_mixed_spacing."""
+    graph replay.  The identity-sweep flavours' bits (lean, lean with the block test,
+    general) are checked the same way by test_per_simulation_sweep_counts_mixed_spacing."""
     from pyheartsim_amd import runtime
     wa = small_shell(t_final=20.0, seed=3)
     wb = small_shell(t_final=30.0, seed=5, semi_axes=(8.0, 7.0, 6.0), n_scar=1)
@@ -189,13 +189,18 @@ def test_dropin_compute_matches_oracle(cuda, precision):
         assert np.max(np.abs(u - ref["u"])) <= 5e-2
 
 
-def test_per_simulation_sweep_counts_mixed_spacing(cuda):
+@pytest.mark.parametrize("flavour", ["lean", "flags", "general"])
+def test_per_simulation_sweep_counts_mixed_spacing(cuda, flavour):
     """Automatic sweeps: a 0.25 mm shell (lambda_max ~ 20) needs more sweeps than a 0.5 mm
     one (~ 6).  Packed together the arena launches the larger count, and each simulation
     still computes bitwise what it computes alone (identity sweeps past its own count);
-    the fine one stays within 1 step of the FP64 oracle's LAT."""
+    the fine one stays within 1 step of the FP64 oracle's LAT.  The coarse shell has no
+    temporary block (lean kernels with identity sweeps), a temporary block (the same with
+    the block test) or per-voxel conductivity (general kernels)."""
     from pyheartsim_amd import runtime
-    coarse = small_shell(t_final=20.0, seed=3)
+    coarse = small_shell(t_final=20.0, seed=3, temp_block=flavour == "flags")
+    if flavour == "general":
+        coarse = _hetero_c(coarse)
     fine = small_shell(delta=0.25, semi_axes=(6.0, 5.5, 5.0), wall=1.5, t_final=20.0, seed=4)
     pc, pf = _prep(coarse), _prep(fine)
     solver = runtime.SolverConfig()           # sweeps = 0: automatic
