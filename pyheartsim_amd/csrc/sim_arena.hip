@@ -877,10 +877,18 @@ int phs_arena_compact(phs_arena* a, double max_waste, int32_t* moved, void* stre
       const size_t bytes = sizeof(BrickRec) * (size_t)n;
       if (next + n > h.brick_off) {  // the ranges overlap: copy through a scratch range
         if (a->move_bytes < bytes) {
+          // copies queued on `s` may still read the old scratch range
+          const hipError_t w = a->move_tmp ? hipStreamSynchronize(s) : hipSuccess;
           hipFree(a->move_tmp);
           a->move_tmp = nullptr;
           a->move_bytes = 0;
-          PHS_TRY_ALLOC(a->move_tmp, bytes);
+          const hipError_t e = w != hipSuccess ? w : hipMalloc(&a->move_tmp, bytes);
+          if (e != hipSuccess) {
+            recompute_high(a);  // the moves made so far stand
+            if (moved) *moved = n_moved;
+            return phs::fail(PHS_ERR_OOM, std::string("compaction scratch: ") +
+                                              hipGetErrorString(e));
+          }
           a->move_bytes = bytes;
         }
         PHS_HIP_CHECK(hipMemcpyAsync(a->move_tmp, a->bricks + h.brick_off, bytes,
