@@ -983,7 +983,19 @@ def run(argv=None) -> dict:
         del src, dst
     fn = {"batched": run_batched, "shell": run_shell, "precision": run_precision,
           "egm": run_egm, "serve": run_serve}[args.workload]
-    return fn(args, torch, runtime, pre)
+    rec = fn(args, torch, runtime, pre)
+    rec["device"] = device_info(torch)
+    return rec
+
+
+def device_info(torch) -> dict:
+    """The GPU the line was measured on, as the runtime reports it (name, architecture,
+    compute units, memory), and the HIP and PyTorch versions."""
+    p = torch.cuda.get_device_properties(0)
+    return dict(name=p.name, arch=getattr(p, "gcnArchName", None),
+                compute_units=p.multi_processor_count, memory_gib=p.total_memory / 2**30,
+                l2_cache_mib=getattr(p, "L2_cache_size", 0) / 2**20 or None,
+                hip=torch.version.hip, torch=torch.__version__)
 
 
 def main():

@@ -26,6 +26,7 @@ def test_bench_driver_command_reduced(cuda):
     assert abs(line["value"] - line["config"]["voxels"] / (line["ms_per_step"] * 1e-3)) \
         <= 1e-6 * line["value"]
     assert "workload" in line["config"]
+    assert line["device"]["compute_units"] > 0 and "gfx950" in (line["device"]["arch"] or "")
     roof = line["roofline"]
     assert roof["bound"] == "hbm" and 0 < roof["frac"] < 1.5
     assert roof["achieved"] == pytest.approx(roof["frac"] * roof["peak"])
