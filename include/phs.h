@@ -207,12 +207,16 @@ int phs_arena_remove(phs_arena* a, int32_t slot, void* stream);
 /* Launches cover the bricks up to the highest resident simulation's, so removals leave holes
  * that are launched for nothing.  When the launched bricks exceed (1 + max_waste) times the
  * resident ones, slides the resident simulations' brick records down in brick order until
- * they are contiguous from brick 0.  Voxel data do not move (a brick record holds voxel
- * indices only), so every simulation computes bitwise what it computed before; the copies
- * are ordered on `stream` (a move onto part of its old range goes through a device scratch
- * range, allocated on first need).  *moved (may be NULL) receives the number of simulations
- * moved.  Replaces nothing in the reference, whose simulations do not share a device. */
-int phs_arena_compact(phs_arena* a, double max_waste, int32_t* moved, void* stream);
+ * they are contiguous from brick 0.  With `voxels` != 0 it first slides their voxel data down
+ * in voxel order the same way (the state arrays, rebasing the records' voxel indices and the
+ * slots' offsets), so that the free voxels form one range: an add that failed with
+ * PHS_ERR_CAPACITY although enough voxels were free in total then fits.  Every simulation
+ * computes bitwise what it computed before; the copies are ordered on `stream` (a move onto
+ * part of its old range goes through a device scratch range, allocated on first need).
+ * *moved (may be NULL) receives the number of range moves.  Replaces nothing in the
+ * reference, whose simulations do not share a device. */
+int phs_arena_compact(phs_arena* a, double max_waste, int32_t voxels, int32_t* moved,
+                      void* stream);
 /* Advance every resident simulation by n_steps steps (each stops at its own n_steps).
  * Replaces the loop simulation_gpu.py:358-407. */
 int phs_arena_step(phs_arena* a, int32_t n_steps, void* stream);

@@ -114,7 +114,7 @@ SIGNATURES = {
     "phs_arena_create": (C.c_int, [C.POINTER(ArenaConfig), C.POINTER(_P)]),
     "phs_arena_add": (C.c_int, [_P, C.POINTER(SimDesc), C.POINTER(Outputs), _PI32, _P]),
     "phs_arena_remove": (C.c_int, [_P, _I32, _P]),
-    "phs_arena_compact": (C.c_int, [_P, _D, _PI32, _P]),
+    "phs_arena_compact": (C.c_int, [_P, _D, _I32, _PI32, _P]),
     "phs_arena_step": (C.c_int, [_P, _I32, _P]),
     "phs_arena_step_graph": (C.c_int, [_P, _I32, _P]),
     "phs_arena_graph_prepare": (C.c_int, [_P, _I32]),

@@ -801,6 +801,22 @@ __global__ void k_invalidate(BrickRec* bricks, int64_t b0, int64_t nb) {
   if (i < nb) bricks[b0 + i].slot = -1;
 }
 
+// A simulation's voxel range moved by delta: its brick records' voxel indices (own start and
+// gather bases) move with it.  A gather base of a face without a neighbour brick is never
+// read (no lane of that face gathers), so it may move with the others.
+__global__ void k_rebase(BrickRec* bricks, int64_t b0, int64_t nb, int32_t delta) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  BrickRec& r = bricks[b0 + i];
+  r.start += delta;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) r.nstart[k] += delta;
+}
+
+__global__ void k_set_voxel_off(SlotRec* slots, int slot, int64_t off) {
+  if (threadIdx.x == 0) slots[slot].voxel_off = off;
+}
+
 // Diagnostic: sum of r^2 and b^2 with b = u* + alpha sum(u*_m - u*_n), x = u* + dacc,
 // r = b - (x - alpha sum(x_m - x_n)), for the last completed step of one slot.
 template <int BX, int BY, int BZ>

@@ -849,73 +849,138 @@ int phs_arena_remove(phs_arena* a, int32_t slot, void* stream) {
 // Launches cover bricks [0, brick_high), so the ranges that finished simulations leave below
 // the highest resident one are launched for nothing.  A brick record holds voxel indices only
 // (its own start and its neighbours' gather bases, all in its simulation's voxel range), never
-// another brick's index, so a simulation's records can move to any brick range as they are:
-// its voxel data stay where they are.  Compaction slides the simulations down in brick order
-// until they are contiguous from brick 0.
-int phs_arena_compact(phs_arena* a, double max_waste, int32_t* moved, void* stream) {
+// another brick's index, so a simulation's records can move to any brick range as they are.
+// Compaction slides the simulations down in brick order until they are contiguous from brick
+// 0; with `voxels` it also slides their voxel data down in voxel order (so that the free
+// voxels form one range an admission can use), rebasing the records' voxel indices and the
+// slot's offset.  Per-step scratch arrays (R0, DA, DB) hold nothing between steps and stay.
+}  // extern "C"
+
+namespace {
+
+// `bytes` of the arena's move scratch; copies queued on `s` may still read the old range.
+hipError_t move_scratch(phs_arena* a, size_t bytes, hipStream_t s) {
+  if (a->move_bytes >= bytes) return hipSuccess;
+  const hipError_t w = a->move_tmp ? hipStreamSynchronize(s) : hipSuccess;
+  hipFree(a->move_tmp);
+  a->move_tmp = nullptr;
+  a->move_bytes = 0;
+  if (w != hipSuccess) return w;
+  const hipError_t e = hipMalloc(&a->move_tmp, bytes);
+  if (e == hipSuccess) a->move_bytes = bytes;
+  return e;
+}
+
+// Elements [from, from + n) of `base` to [to, to + n), to < from, through the scratch when
+// the ranges overlap.
+hipError_t move_down(phs_arena* a, void* base, size_t elem, int64_t from, int64_t to, int64_t n,
+                     hipStream_t s) {
+  char* b = (char*)base;
+  const size_t bytes = elem * (size_t)n;
+  if (to + n <= from)
+    return hipMemcpyAsync(b + elem * to, b + elem * from, bytes, hipMemcpyDeviceToDevice, s);
+  hipError_t e = move_scratch(a, bytes, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(a->move_tmp, b + elem * from, bytes, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b + elem * to, a->move_tmp, bytes, hipMemcpyDeviceToDevice, s);
+  return e;
+}
+
+// Slides the resident simulations' ranges (brick or voxel) down until they are contiguous
+// from 0; `off`/`len` pick the range of a slot, `move` moves one slot to a lower offset.
+template <typename Off, typename Len, typename Move>
+int slide(phs_arena* a, RangeAlloc& alloc, Off off, Len len, Move move, int32_t& n_moved) {
+  std::vector<int> order;
+  for (int i = 0; i < a->cfg.max_sims; ++i)
+    if (a->sh[i].used) order.push_back(i);
+  std::sort(order.begin(), order.end(), [&](int x, int y) { return off(x) < off(y); });
+  int64_t next = 0;  // [0, next) holds the simulations already placed
+  for (int i : order) {
+    const int64_t n = len(i), old = off(i);
+    if (old > next) {
+      if (int rc = move(i, next)) return rc;
+      // [0, next) is full, so once the old range is free the lowest range that fits n
+      // starts at next
+      alloc.release(old, n);
+      if (alloc.alloc(n) != next) return phs::fail(PHS_ERR_STATE, "ranges out of order");
+      ++n_moved;
+    }
+    next += n;
+  }
+  return PHS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phs_arena_compact(phs_arena* a, double max_waste, int32_t voxels, int32_t* moved,
+                      void* stream) {
   if (!a) return phs::fail(PHS_ERR_ARG, "handle is NULL");
   if (!(max_waste >= 0.0)) return phs::fail(PHS_ERR_ARG, "max_waste must be >= 0");
   hipStream_t s = (hipStream_t)stream;
   if (moved) *moved = 0;
-  std::vector<int> order;
-  int64_t live = 0;
-  for (int i = 0; i < a->cfg.max_sims; ++i)
-    if (a->sh[i].used) {
-      order.push_back(i);
-      live += a->sh[i].n_bricks;
-    }
-  const int64_t launched = (int64_t)grid_for(a) * kWaves * nb_per_wave(a);
-  if (order.empty() || (double)launched <= (1.0 + max_waste) * (double)live) return PHS_OK;
-  std::sort(order.begin(), order.end(),
-            [&](int x, int y) { return a->sh[x].brick_off < a->sh[y].brick_off; });
   int32_t n_moved = 0;
-  int64_t next = 0;  // bricks [0, next) hold the simulations already placed
-  for (int i : order) {
-    SlotHost& h = a->sh[i];
-    const int64_t n = h.n_bricks;
-    if (h.brick_off > next) {
-      const size_t bytes = sizeof(BrickRec) * (size_t)n;
-      if (next + n > h.brick_off) {  // the ranges overlap: copy through a scratch range
-        if (a->move_bytes < bytes) {
-          // copies queued on `s` may still read the old scratch range
-          const hipError_t w = a->move_tmp ? hipStreamSynchronize(s) : hipSuccess;
-          hipFree(a->move_tmp);
-          a->move_tmp = nullptr;
-          a->move_bytes = 0;
-          const hipError_t e = w != hipSuccess ? w : hipMalloc(&a->move_tmp, bytes);
-          if (e != hipSuccess) {
-            recompute_high(a);  // the moves made so far stand
-            if (moved) *moved = n_moved;
-            return phs::fail(PHS_ERR_OOM, std::string("compaction scratch: ") +
-                                              hipGetErrorString(e));
-          }
-          a->move_bytes = bytes;
-        }
-        PHS_HIP_CHECK(hipMemcpyAsync(a->move_tmp, a->bricks + h.brick_off, bytes,
-                                     hipMemcpyDeviceToDevice, s));
-        PHS_HIP_CHECK(hipMemcpyAsync(a->bricks + next, a->move_tmp, bytes,
-                                     hipMemcpyDeviceToDevice, s));
-        const int64_t tail = h.brick_off + n - (next + n);  // old bricks above the new range
-        k_invalidate<<<(unsigned)((tail + 255) / 256), 256, 0, s>>>(a->bricks, next + n, tail);
-      } else {
-        PHS_HIP_CHECK(hipMemcpyAsync(a->bricks + next, a->bricks + h.brick_off, bytes,
-                                     hipMemcpyDeviceToDevice, s));
-        k_invalidate<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a->bricks, h.brick_off, n);
-      }
-      PHS_HIP_CHECK(hipGetLastError());
-      // [0, next) is full, so once the old range is free the lowest range that fits n
-      // starts at next
-      a->brk.release(h.brick_off, n);
-      const int64_t at = a->brk.alloc(n);
-      if (at != next) return phs::fail(PHS_ERR_STATE, "brick ranges out of order");
-      h.brick_off = next;
-      ++n_moved;
-    }
-    next = h.brick_off + n;
+  int rc = PHS_OK;
+  if (voxels) {
+    rc = slide(
+        a, a->vox, [&](int i) { return a->sh[i].voxel_off; },
+        [&](int i) { return (int64_t)a->sh[i].n_voxel; },
+        [&](int i, int64_t to) -> int {
+          SlotHost& h = a->sh[i];
+          const int64_t from = h.voxel_off, n = h.n_voxel;
+          // the arrays that carry a simulation from one step to the next
+          const struct {
+            void* p;
+            size_t elem;
+          } arr[] = {{a->U, 4},     {a->H, 4},     {a->D1, 4},    {a->D2, 4},
+                     {a->ALPHA, 4}, {a->VF, 1},    {a->DACC, 8},  {a->USTAR, 4},
+                     {h.hparams ? a->PV[0] : nullptr, 4}, {h.hparams ? a->PV[1] : nullptr, 4},
+                     {h.hparams ? a->PV[2] : nullptr, 4}, {h.hparams ? a->PV[3] : nullptr, 4},
+                     {h.hparams ? a->PV[4] : nullptr, 4}};
+          for (const auto& x : arr)
+            if (x.p) {
+              const hipError_t e = move_down(a, x.p, x.elem, from, to, n, s);
+              if (e != hipSuccess)
+                return phs::fail(PHS_ERR_HIP, std::string("voxel move: ") + hipGetErrorString(e));
+            }
+          k_rebase<<<(unsigned)((h.n_bricks + 255) / 256), 256, 0, s>>>(
+              a->bricks, h.brick_off, h.n_bricks, (int32_t)(to - from));
+          k_set_voxel_off<<<1, 64, 0, s>>>(a->slots, i, to);
+          PHS_HIP_CHECK(hipGetLastError());
+          h.voxel_off = to;
+          return PHS_OK;
+        },
+        n_moved);
   }
-  recompute_high(a);
+  int64_t live = 0;
+  for (const auto& h : a->sh)
+    if (h.used) live += h.n_bricks;
+  const int64_t launched = (int64_t)grid_for(a) * kWaves * nb_per_wave(a);
+  if (rc == PHS_OK && live > 0 && (double)launched > (1.0 + max_waste) * (double)live) {
+    rc = slide(
+        a, a->brk, [&](int i) { return a->sh[i].brick_off; },
+        [&](int i) { return (int64_t)a->sh[i].n_bricks; },
+        [&](int i, int64_t to) -> int {
+          SlotHost& h = a->sh[i];
+          const int64_t from = h.brick_off, n = h.n_bricks;
+          const hipError_t e = move_down(a, a->bricks, sizeof(BrickRec), from, to, n, s);
+          if (e != hipSuccess)
+            return phs::fail(PHS_ERR_HIP, std::string("brick move: ") + hipGetErrorString(e));
+          // the old records not overwritten by the new range
+          const int64_t lo = std::max(from, to + n);
+          k_invalidate<<<(unsigned)((from + n - lo + 255) / 256), 256, 0, s>>>(a->bricks, lo,
+                                                                                from + n - lo);
+          PHS_HIP_CHECK(hipGetLastError());
+          h.brick_off = to;
+          return PHS_OK;
+        },
+        n_moved);
+  }
+  recompute_high(a);  // the moves made stand, also after a failure
   if (moved) *moved = n_moved;
-  return PHS_OK;
+  return rc;
 }
 
 int phs_arena_step(phs_arena* a, int32_t n_steps, void* stream) {

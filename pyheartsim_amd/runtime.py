@@ -355,15 +355,23 @@ class Arena:
                    "phs_arena_remove")
         self.slots.pop(slot, None)
 
-    def compact(self, max_waste: float = 0.05, stream=None) -> int:
+    def compact(self, max_waste: float = 0.05, voxels: bool = False, stream=None) -> int:
         """When the launches cover more than (1 + max_waste) times the resident bricks,
-        slide the simulations' brick records down until they are contiguous
-        (phs_arena_compact); returns the number of simulations moved.  Results are
-        unchanged, bit for bit."""
+        slide the simulations' brick records down until they are contiguous; with `voxels`,
+        first slide their voxel data down too, so that the free voxels form one range
+        (phs_arena_compact).  Returns the number of range moves.  Results are unchanged,
+        bit for bit."""
         moved = C.c_int32()
-        _lib.check(self.lib.phs_arena_compact(self.handle, float(max_waste), C.byref(moved),
-                                              _lib.stream_ptr(stream)), "phs_arena_compact")
+        _lib.check(self.lib.phs_arena_compact(self.handle, float(max_waste), 1 if voxels else 0,
+                                              C.byref(moved), _lib.stream_ptr(stream)),
+                   "phs_arena_compact")
         return moved.value
+
+    def free_capacity(self):
+        """(free voxels, free bricks, free slots) of the arena, summed over its free ranges."""
+        c = self.counts()
+        return (self.cfg.max_voxels - c["voxels"], self.cfg.max_bricks - c["bricks"],
+                self.cfg.max_sims - len(self.slots))
 
     def step(self, n: int, graph: bool = False, stream=None):
         fn = self.lib.phs_arena_step_graph if graph else self.lib.phs_arena_step

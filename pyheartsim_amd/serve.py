@@ -181,18 +181,33 @@ class Server:
         fp = r.footprint_bytes(2 * self.ring)
         if self.in_use + fp > self.budget:
             return None
-        try:
-            slot = self.arena.add(r.prep, samples=True, ring=2 * self.ring)
-        except runtime._lib.PhsError as e:
-            if e.code == runtime._lib.PHS_ERR_CAPACITY:  # no free slot, voxels or bricks
+        slot = self._add(r)
+        if slot is None:
+            free_voxels, _, free_slots = self.arena.free_capacity()
+            if free_slots == 0 or free_voxels < r.prep.n_voxel:
                 return None
-            raise
+            # enough voxels in total, split by the ranges of running simulations: close the
+            # gaps (results unchanged) and try once more
+            self.arena.compact(0.0, voxels=True)
+            slot = self._add(r)
+            if slot is None:
+                return None
         perm = self.arena.slots[slot].perm
         lf = LeadField(r.electrodes, r.voxel, None, r.c_voxel, r.delta, r.neighbor_id_2d, perm=perm)
         egm = self.torch.empty((r.prep.n_saved(), len(r.electrodes)), dtype=self.torch.float32,
                                device="cuda")
         self.in_use += fp
         return _Active(r, slot, lf, egm, 0, now)
+
+    def _add(self, r: Request):
+        """The request's arena slot, or None when the arena has no room (no free slot, or no
+        free voxel or brick range large enough)."""
+        try:
+            return self.arena.add(r.prep, samples=True, ring=2 * self.ring)
+        except runtime._lib.PhsError as e:
+            if e.code == runtime._lib.PHS_ERR_CAPACITY:
+                return None
+            raise
 
     def _contract(self, a: _Active, upto: int):
         """Electrogram rows [done_rows, upto) from the sample ring (upto - done_rows <= ring)."""

@@ -155,6 +155,48 @@ def test_compaction_moves_records_bitwise(cuda, removed):
     ar.close()
 
 
+def test_voxel_compaction_admits_into_freed_ranges(cuda):
+    """An arena exactly as large as three simulations: with the outer two removed, a fourth
+    that fits in neither freed voxel range is refused (PHS_ERR_CAPACITY).  Compacting with
+    voxels moves the middle one's voxel data down, the fourth then fits, and both compute
+    bitwise what they compute alone."""
+    from pyheartsim_amd import _lib, runtime
+    axes = {"a": (6.5, 5.5, 5.0), "b": (9.0, 8.0, 7.0), "c": (7.0, 6.0, 5.0),
+            "d": (7.5, 6.5, 5.5)}
+    ps = {k: _prep(small_shell(t_final=10.0 if k in "ac" else 30.0, seed=3 + i, semi_axes=ax))
+          for i, (k, ax) in enumerate(axes.items())}
+    n = {k: p.n_voxel for k, p in ps.items()}
+    assert max(n["a"], n["c"]) < n["d"] <= n["a"] + n["c"]
+    solver = runtime.SolverConfig()
+    ref = {}
+    for k in "bd":
+        one = runtime.Arena(n[k], 1, solver)
+        s = one.add(ps[k])
+        one.step(ps[k].n_steps)
+        ref[k] = [t.cpu().numpy() for t in one.outputs(s) if t is not None]
+        one.close()
+    ar = runtime.Arena(n["a"] + n["b"] + n["c"], 3, solver)
+    sa, sb, sc = ar.add(ps["a"]), ar.add(ps["b"]), ar.add(ps["c"])
+    ar.step(ps["a"].n_steps, graph=True)
+    ar.remove(sa)
+    ar.remove(sc)
+    assert ar.free_capacity()[0] >= n["d"]
+    with pytest.raises(_lib.PhsError) as e:
+        ar.add(ps["d"])
+    assert e.value.code == _lib.PHS_ERR_CAPACITY
+    assert ar.compact(0.0, voxels=True) >= 1
+    assert ar.slot_info(sb)["voxel_offset"] == 0
+    sd = ar.add(ps["d"])
+    for _ in range(ps["d"].n_steps // 50 + 1):
+        ar.step(50, graph=True)
+    for k, s in (("b", sb), ("d", sd)):
+        got = [t.cpu().numpy() for t in ar.outputs(s) if t is not None]
+        assert len(got) == len(ref[k]) == 2
+        for g, r in zip(got, ref[k]):
+            assert np.array_equal(g, r), k
+    ar.close()
+
+
 def test_fp32_arena_temp_block_and_scar_vs_oracle(cuda):
     from pyheartsim_amd import runtime
     w = small_shell(t_final=40.0, n_scar=2, temp_block=True)

@@ -91,3 +91,31 @@ def test_full_arena_reports_capacity_and_server_queues(cuda):
                        chunk_steps=50)
     st = srv.run(reqs)
     assert st.completed == len(reqs)
+
+
+def test_server_compacts_voxels_to_admit(cuda):
+    """A served arena exactly as large as its first three requests: once the short outer two
+    finish, the fourth fits in neither freed voxel range, so the server compacts the voxel
+    ranges and admits it while the long middle one still runs.  The fourth's electrogram and
+    activation times equal those it gets served alone, bit for bit."""
+    from pyheartsim_amd import serve
+
+    def req(rid, n_target, seed, duration):
+        return serve.build_request(dict(rid=rid, arrival=0.0, n_target=n_target, seed=seed,
+                                        model=0, duration_ms=duration, n_electrodes=8,
+                                        delta=0.4, dt_ms=0.05))
+    a, b, c, d = req(0, 2e4, 2, 10.0), req(1, 6e4, 4, 60.0), req(2, 2.5e4, 6, 10.0), \
+        req(3, 3e4, 3, 20.0)
+    n = [r.prep.n_voxel for r in (a, b, c, d)]
+    assert max(n[0], n[2]) < n[3] <= n[0] + n[2]
+    srv = serve.Server(max_voxels=n[0] + n[1] + n[2], max_sims=4, hbm_budget_bytes=8e9,
+                       ring=16, chunk_steps=50, keep_results=True)
+    st = srv.run([a, b, c, d])
+    assert st.completed == 4
+    rec = {r["rid"]: r for r in st.records}
+    assert rec[3]["admitted"] < rec[1]["done"]     # admitted beside the running middle one
+    alone = serve.Server(max_voxels=n[3], max_sims=1, hbm_budget_bytes=8e9, ring=16,
+                         chunk_steps=50, keep_results=True)
+    ra = alone.run([serve.Request(**{**d.__dict__, "rid": 9})]).records[0]
+    assert np.array_equal(ra["egm"].numpy(), rec[3]["egm"].numpy())
+    assert np.array_equal(ra["lat"].numpy(), rec[3]["lat"].numpy())
