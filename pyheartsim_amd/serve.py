@@ -5,7 +5,8 @@ electrode set) arrive open-loop at Poisson times.  The loop
 
   * admits queued requests first-come-first-served whenever the arena has room for
     their voxels and bricks and the HBM budget has room for their footprint
-    (state, sample ring, lead field, activation times, electrogram);
+    (state, sample ring, lead field, activation times, electrogram); when the free
+    voxels suffice only in total, it first compacts the arena's voxel ranges;
   * keeps every admitted simulation's state on the device and advances all of them
     together in hipGraph-replayed chunks of steps (each simulation on its own step
     counter, so a request admitted mid-run starts at its step 0);
