@@ -581,6 +581,7 @@ def serve_leg(torch, runtime, solver, reqs, qps, max_voxels=60_000_000):
     ns = [r.prep.n_voxel for r in reqs]
     return dict(qps=qps, requests=st.completed, p50_s=st.p50, p99_s=st.p99,
                 voxel_steps_per_s=st.throughput, wall_s=st.wall,
+                compaction_moves=dict(bricks=st.brick_moves, voxels=st.voxel_moves),
                 n_voxel_range=[min(ns), max(ns)],
                 duration_ms_range=[min(r.duration_ms for r in reqs),
                                    max(r.duration_ms for r in reqs)],

@@ -157,6 +157,8 @@ class ServeStats:
     p99: float
     throughput: float
     records: list
+    brick_moves: int = 0    # simulations whose brick records compaction moved
+    voxel_moves: int = 0    # simulations whose voxel data compaction moved
 
 
 class Server:
@@ -171,6 +173,8 @@ class Server:
         self.budget = hbm_budget_bytes
         self.in_use = 0
         self.keep = keep_results
+        self.brick_moves = 0
+        self.voxel_moves = 0
 
     def _admit(self, r: Request, now: float) -> _Active | None:
         # A chunk of steps must produce at most `ring` samples: the ring holds two halves,
@@ -189,7 +193,7 @@ class Server:
                 return None
             # enough voxels in total, split by the ranges of running simulations: close the
             # gaps (results unchanged) and try once more
-            self.arena.compact(0.0, voxels=True)
+            self.voxel_moves += self.arena.compact(0.0, voxels=True)
             slot = self._add(r)
             if slot is None:
                 return None
@@ -299,7 +303,7 @@ class Server:
             if removed:
                 # close the ranges just freed, so that the launches cover the resident
                 # bricks rather than the arena's high-water mark
-                self.arena.compact()
+                self.brick_moves += self.arena.compact()
             if not active and retiring:
                 time.sleep(0.0005)
         torch.cuda.synchronize()
@@ -308,7 +312,7 @@ class Server:
         return ServeStats(lat, voxel_steps, wall, len(records), len(records),
                           float(np.percentile(la, 50)) if len(la) else float("nan"),
                           float(np.percentile(la, 99)) if len(la) else float("nan"),
-                          voxel_steps / wall, records)
+                          voxel_steps / wall, records, self.brick_moves, self.voxel_moves)
 
     def _finish(self, a: _Active):
         torch = self.torch

@@ -114,6 +114,7 @@ def test_server_compacts_voxels_to_admit(cuda):
     assert st.completed == 4
     rec = {r["rid"]: r for r in st.records}
     assert rec[3]["admitted"] < rec[1]["done"]     # admitted beside the running middle one
+    assert st.voxel_moves >= 1
     alone = serve.Server(max_voxels=n[3], max_sims=1, hbm_budget_bytes=8e9, ring=16,
                          chunk_steps=50, keep_results=True)
     ra = alone.run([serve.Request(**{**d.__dict__, "rid": 9})]).records[0]
