@@ -308,25 +308,32 @@ struct Stencil {
     return s;
   }
 
-  // The same sum formed as (sum of the six m) - 6 v in T: one conversion and one add per
-  // direction instead of a conversion, a subtraction and an add.  For T = double and V = float
-  // the rounding is ~1e-16 of |v|, far below the FP32 rounding of the right-hand side it
-  // feeds.
+  // The same sum formed as (sum of terms) - n v in T, with one select per in-brick direction
+  // only.  An axis gather reads the cross neighbour on a face lane that has one and the lane's
+  // own value otherwise (a lane lies on at most one face of an axis of extent > 1), so
+  // (g sc - v) is the axis' cross term as it stands: g sc and v are the same product on a lane
+  // without one.  The in-brick terms select the neighbour lane or v.  n counts the terms
+  // (9 for 4 x 4 x 4 bricks).  For T = double and V = float the rounding is ~1e-16 of |v|,
+  // far below the FP32 rounding of the right-hand side it feeds.
   template <typename T, typename V>
   __device__ __forceinline__ T sum_total(V v, const V (&g)[6], V sc, int lane) const {
     const V pv[6] = {exch<0>(v, lane), exch<1>(v, lane), exch<2>(v, lane),
                      exch<3>(v, lane), exch<4>(v, lane), exch<5>(v, lane)};
-    T s;
+    T s = (T)(g[0] * sc);
+    constexpr int n = 3 + both(0) + both(1) + both(2) + 2 * ((SH::B[0] > 1) + (SH::B[1] > 1) +
+                                                          (SH::B[2] > 1));
 #pragma unroll
-    for (int d = 0; d < 6; ++d) {
-      V m = lane_bit(pcr[d]) ? g[d] * sc : v;
-      m = lane_bit(pin[d]) ? pv[d] : m;
-      s = d == 0 ? (T)m : s + (T)m;
+    for (int ax = 0; ax < 3; ++ax) {
+      if (ax > 0) s += (T)(g[2 * ax] * sc);
+      if (both(ax)) s += (T)(g[2 * ax + 1] * sc);
     }
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+      if (SH::B[d >> 1] > 1) s += (T)(lane_bit(pin[d]) ? pv[d] : v);
     if constexpr (sizeof(T) == 4)
-      return __builtin_fmaf(-6.0f, (float)v, s);
+      return __builtin_fmaf(-(float)n, (float)v, s);
     else
-      return __builtin_fma(-6.0, (double)v, s);
+      return __builtin_fma(-(double)n, (double)v, s);
   }
 };
 
