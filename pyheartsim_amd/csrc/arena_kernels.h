@@ -731,8 +731,9 @@ __global__ void __launch_bounds__(kBlock) k_sweep(ArenaDev A, int k, int K, cons
     const float inv_theta = sv[j].inv_theta;
     const float sc = FIRST ? inv_theta : 1.0f;
     const float d = st[j].valid ? dn[j] * sc : 0.0f;
-    // d is the CN correction, so the rounding of sum(m) - 6 d (a few ulp of |d|) stays far
-    // below the sweep's own truncation error; this form takes five fewer VALU instructions
+    // d is the CN correction, so the rounding of (sum of terms) - n d (a few ulp of |d|)
+    // stays far below the sweep's own truncation error; this form takes fewer VALU
+    // instructions than the sum of differences
     const float s = st[j].template sum_total<float>(d, g6[j], sc, lane);
     if (!st[j].valid || !sv[j].live) continue;
     const int i = st[j].idx;
