@@ -308,6 +308,32 @@ struct Stencil {
     return s;
   }
 
+  // 2 sum(u_m - u_n) + sum(g_m - g_n) over present neighbours in one pass: per direction
+  // fma(2, u_m - u_n, g_m - g_n) of FP32 differences, each exact or rounded relative to
+  // itself.  An axis gather reads the cross neighbour on a face lane that has one and the
+  // lane's own value otherwise (a lane lies on at most one face of an axis of extent > 1), so
+  // its differences enter with no select (they are 0 on a lane without one); the in-brick
+  // terms are selected by presence.
+  template <typename V>
+  __device__ __forceinline__ float sum_pair(V u, const V (&gu)[6], V g, const V (&gg)[6],
+                                            int lane) const {
+    const V pu[6] = {exch<0>(u, lane), exch<1>(u, lane), exch<2>(u, lane),
+                     exch<3>(u, lane), exch<4>(u, lane), exch<5>(u, lane)};
+    const V pg[6] = {exch<0>(g, lane), exch<1>(g, lane), exch<2>(g, lane),
+                     exch<3>(g, lane), exch<4>(g, lane), exch<5>(g, lane)};
+    float s = __builtin_fmaf(2.0f, gu[0] - u, gg[0] - g);
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      if (ax > 0) s += __builtin_fmaf(2.0f, gu[2 * ax] - u, gg[2 * ax] - g);
+      if (both(ax)) s += __builtin_fmaf(2.0f, gu[2 * ax + 1] - u, gg[2 * ax + 1] - g);
+    }
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+      if (SH::B[d >> 1] > 1)
+        s += lane_bit(pin[d]) ? __builtin_fmaf(2.0f, pu[d] - u, pg[d] - g) : 0.0f;
+    return s;
+  }
+
   // The same sum formed as (sum of terms) - n v in T, with one select per in-brick direction
   // only.  An axis gather reads the cross neighbour on a face lane that has one and the lane's
   // own value otherwise (a lane lies on at most one face of an axis of extent > 1), so
@@ -608,16 +634,17 @@ __global__ void __launch_bounds__(kBlock) k_rhs(ArenaDev A) {
   for (int j = 0; j < NB; ++j) {
     if (!on[j]) continue;
     const float uv = st[j].valid ? un[j] : 0.0f;
-    const float s = st[j].template sum<float>(uv, gu[j], 1.0f, lane);
-    float sg = 0.0f, gn = 0.0f;
+    // r0 = a (2 sum(u*) + sum(d0)) - d0: the two neighbour sums in one pass
+    float s2, gn = 0.0f;
     if (GUESS >= 1) {
-      const float gv = st[j].valid ? gn0[j] : 0.0f;
-      sg = st[j].template sum<float>(gv, gg[j], 1.0f, lane);
-      gn = gv;
+      gn = st[j].valid ? gn0[j] : 0.0f;
+      s2 = st[j].sum_pair(uv, gu[j], gn, gg[j], lane);
+    } else {
+      s2 = 2.0f * st[j].template sum<float>(uv, gu[j], 1.0f, lane);
     }
     if (!st[j].valid || !sv[j].live) continue;
     const float a = alpha_at<float, FLAV>(sv[j], ain[j]);
-    stv(A.R0, st[j].idx, 2.0f * a * s - (gn - a * sg));
+    stv(A.R0, st[j].idx, a * s2 - gn);
   }
   keep_alive(pfr);
   // The first workgroup's first wavefront also stages every slot's next-step values.  This
