@@ -155,11 +155,13 @@ def test_compaction_moves_records_bitwise(cuda, removed):
     ar.close()
 
 
-def test_voxel_compaction_admits_into_freed_ranges(cuda):
+@pytest.mark.parametrize("refine", [0, 1])
+def test_voxel_compaction_admits_into_freed_ranges(cuda, refine):
     """An arena exactly as large as three simulations: with the outer two removed, a fourth
     that fits in neither freed voxel range is refused (PHS_ERR_CAPACITY).  Compacting with
     voxels moves the middle one's voxel data down, the fourth then fits, and both compute
-    bitwise what they compute alone."""
+    bitwise what they compute alone (refine 1: with the FP64 refinement accumulator and the
+    diagnostic u* arrays, which move too; the moved one's CN residual is recomputed)."""
     from pyheartsim_amd import _lib, runtime
     axes = {"a": (6.5, 5.5, 5.0), "b": (9.0, 8.0, 7.0), "c": (7.0, 6.0, 5.0),
             "d": (7.5, 6.5, 5.5)}
@@ -167,7 +169,8 @@ def test_voxel_compaction_admits_into_freed_ranges(cuda):
           for i, (k, ax) in enumerate(axes.items())}
     n = {k: p.n_voxel for k, p in ps.items()}
     assert max(n["a"], n["c"]) < n["d"] <= n["a"] + n["c"]
-    solver = runtime.SolverConfig()
+    solver = runtime.SolverConfig(refine=1, refine_sweeps=8, diag=True) if refine else \
+        runtime.SolverConfig()
     ref = {}
     for k in "bd":
         one = runtime.Arena(n[k], 1, solver)
@@ -194,6 +197,8 @@ def test_voxel_compaction_admits_into_freed_ranges(cuda):
         assert len(got) == len(ref[k]) == 2
         for g, r in zip(got, ref[k]):
             assert np.array_equal(g, r), k
+    if refine:
+        assert ar.last_residual(sb) <= 1e-6   # recomputed from the moved u* and correction
     ar.close()
 
 
