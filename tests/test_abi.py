@@ -44,6 +44,9 @@ def test_argument_errors_need_no_device():
     cfg.sweeps = 1
     assert lib.phs_arena_create(C.byref(cfg), C.byref(h)) == -1
     assert lib.phs_egm_run(None, None, 1, 1, None, 0, None) == -1
+    moved = C.c_int32(7)
+    assert lib.phs_arena_compact(None, 0.0, 1, C.byref(moved), None) == -1
+    assert b"NULL" in lib.phs_last_error()
 
 
 @pytest.mark.parametrize("brick", [(4, 4, 4), (8, 8, 1), (8, 4, 2), (4, 8, 2), (16, 4, 1)])
